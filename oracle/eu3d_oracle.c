@@ -1,0 +1,1672 @@
+/*
+ * eu3d_oracle.c -- CPU restatement of the EU3D reactive-Euler step.
+ *
+ * TEST INFRASTRUCTURE ONLY (the oracle).  Loaded by tests/, by
+ * __graft_entry__.smoke() as the checker, and timed by bench.py's
+ * cpu_baseline leg.  The product library never links it.
+ *
+ * Follows the reference CPU path (use_cuda = false).  Every function cites
+ * the reference file:line it restates.  Floating-point expressions keep the
+ * reference's operand order and association; the reference's pow(x,2) and
+ * pow(x,-1) calls are written x*x and 1.0/x, which is exactly what GCC -O3
+ * emits for them in the reference build (Makefile:6); every other pow/exp/
+ * log call is a libm call in both.  Build with -ffp-contract=off.
+ *
+ * Parity pins (see tests/test_oracle_*.py): MUSCL_2/Minmod, Function::sum,
+ * FileReader and Mesh against the reference's own sources compiled under
+ * oracle/_ref; NASA-9 thermo, AUSM+, EE, rates, trapezoid, Nchem, Newton-T
+ * and ghost maps against closed forms, invariants and brute force.
+ * The coupled multi-step trajectory (Flowfield.cpp + Reaction.cpp + Euler.cpp)
+ * cannot be built here (needs cuda_runtime.h) and is pinned only through its
+ * parts: "parity unpinned" for the whole-trajectory composition.
+ */
+#define _POSIX_C_SOURCE 200809L
+#include "eu3d_oracle.h"
+
+#include <ctype.h>
+#include <errno.h>
+#include <math.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define THERMO_SWITCH_T 1000.0 /* include/ThermoConstants.hpp:3 */
+
+#define IDX(s, i, j, k) ((((size_t)(i)) * (s)->NJ + (size_t)(j)) * (s)->NK + (size_t)(k))
+
+/* ====================================================================== */
+/* FileReader (src/FileReader.cpp:53-84, include/FileReader.hpp:74-111)  */
+/* ====================================================================== */
+
+typedef struct {
+    char key[128];
+    char val[512];
+} or_kv;
+typedef struct {
+    or_kv it[512];
+    int n;
+} or_keyfile;
+
+static void kf_set(or_keyfile *kf, const char *key, const char *val)
+{
+    for (int i = 0; i < kf->n; i++)
+        if (strcmp(kf->it[i].key, key) == 0) {
+            snprintf(kf->it[i].val, sizeof kf->it[i].val, "%s", val);
+            return;
+        }
+    if (kf->n < 512) {
+        snprintf(kf->it[kf->n].key, sizeof kf->it[kf->n].key, "%s", key);
+        snprintf(kf->it[kf->n].val, sizeof kf->it[kf->n].val, "%s", val);
+        kf->n++;
+    }
+}
+
+static const char *kf_get(const or_keyfile *kf, const char *key)
+{
+    for (int i = 0; i < kf->n; i++)
+        if (strcmp(kf->it[i].key, key) == 0)
+            return kf->it[i].val;
+    return NULL;
+}
+
+/* next whitespace-delimited token from *p (std::istream >> std::string) */
+static int next_token(const char **p, char *out, size_t outlen)
+{
+    const char *q = *p;
+    while (*q && isspace((unsigned char)*q))
+        q++;
+    if (!*q) {
+        *p = q;
+        return 0;
+    }
+    size_t n = 0;
+    while (*q && !isspace((unsigned char)*q)) {
+        if (n + 1 < outlen)
+            out[n++] = *q;
+        q++;
+    }
+    out[n] = 0;
+    *p = q;
+    return 1;
+}
+
+/* FileReader::readFile: the first two tokens of each line; both strings
+ * keep their previous values when a line runs out of tokens; numbers are
+ * re-stored through a default-precision stream (6 significant digits). */
+static int read_keyfile(const char *path, or_keyfile *kf)
+{
+    FILE *f = fopen(path, "rb");
+    if (!f)
+        return -1;
+    char line[4096], e1[512] = "", e2[512] = "", tok[512];
+    while (fgets(line, sizeof line, f)) {
+        size_t L = strlen(line);
+        if (L && line[L - 1] == '\n')
+            line[L - 1] = 0;
+        const char *p = line;
+        if (next_token(&p, tok, sizeof tok)) {
+            snprintf(e1, sizeof e1, "%s", tok);
+            if (next_token(&p, tok, sizeof tok))
+                snprintf(e2, sizeof e2, "%s", tok);
+        }
+        if (e1[0] && e2[0] && e1[0] != '#' && e2[0] != '#') {
+            char *end;
+            errno = 0;
+            double v = strtod(e2, &end);
+            if (end != e2 && errno != ERANGE) {
+                char buf[64];
+                snprintf(buf, sizeof buf, "%g", v);
+                kf_set(kf, e1, buf);
+            } else {
+                kf_set(kf, e1, e2);
+            }
+        }
+    }
+    fclose(f);
+    return 0;
+}
+
+static int kf_int(const or_keyfile *kf, const char *key, int *ok)
+{
+    const char *v = kf_get(kf, key);
+    if (!v) {
+        *ok = 0;
+        return 0;
+    }
+    char *end;
+    errno = 0;
+    long r = strtol(v, &end, 10);
+    if (end == v || errno == ERANGE || r > 2147483647L || r < -2147483647L - 1)
+        return 0;
+    return (int)r;
+}
+
+static double kf_double(const or_keyfile *kf, const char *key, int *ok)
+{
+    const char *v = kf_get(kf, key);
+    if (!v) {
+        *ok = 0;
+        return 0;
+    }
+    char *end;
+    errno = 0;
+    double r = strtod(v, &end);
+    if (end == v || errno == ERANGE)
+        return 0;
+    return r;
+}
+
+static double kf_double_default(const or_keyfile *kf, const char *key, double dflt)
+{
+    int ok = 1;
+    if (!kf_get(kf, key))
+        return dflt;
+    return kf_double(kf, key, &ok);
+}
+
+/* ====================================================================== */
+/* positional mechanism / thermo readers (src/Reaction.cpp:156-329)      */
+/* ====================================================================== */
+
+typedef struct {
+    char *text;
+    char **lines;
+    int nlines;
+} or_lines;
+
+static int load_lines(const char *path, or_lines *L)
+{
+    FILE *f = fopen(path, "rb");
+    if (!f)
+        return -1;
+    fseek(f, 0, SEEK_END);
+    long n = ftell(f);
+    fseek(f, 0, SEEK_SET);
+    L->text = (char *)malloc((size_t)n + 1);
+    if (fread(L->text, 1, (size_t)n, f) != (size_t)n) {
+        fclose(f);
+        return -1;
+    }
+    fclose(f);
+    L->text[n] = 0;
+    int cnt = 1;
+    for (long i = 0; i < n; i++)
+        if (L->text[i] == '\n')
+            cnt++;
+    L->lines = (char **)malloc(sizeof(char *) * (size_t)(cnt + 1));
+    L->nlines = 0;
+    char *p = L->text;
+    L->lines[L->nlines++] = p;
+    for (long i = 0; i < n; i++)
+        if (L->text[i] == '\n') {
+            L->text[i] = 0;
+            L->lines[L->nlines++] = L->text + i + 1;
+        }
+    return 0;
+}
+
+static void free_lines(or_lines *L)
+{
+    free(L->text);
+    free(L->lines);
+}
+
+/* token stream starting at line `line` (seek_to_line + operator>>) */
+typedef struct {
+    const or_lines *L;
+    int line;
+    const char *p;
+} or_tokstream;
+
+static void ts_seek(or_tokstream *t, const or_lines *L, int line)
+{
+    t->L = L;
+    t->line = line;
+    t->p = line < L->nlines ? L->lines[line] : "";
+}
+
+static int ts_next(or_tokstream *t, char *out, size_t outlen)
+{
+    for (;;) {
+        if (next_token(&t->p, out, outlen))
+            return 1;
+        t->line++;
+        if (t->line >= t->L->nlines)
+            return 0;
+        t->p = t->L->lines[t->line];
+    }
+}
+
+/* std::stoi on a tab-separated item: leading whitespace skipped, decimal prefix */
+static int item_stoi(const char *s, int *ok)
+{
+    char *end;
+    long v = strtol(s, &end, 10);
+    if (end == s)
+        *ok = 0;
+    return (int)v;
+}
+
+/* stringstream >> int: prefix parse, 0 on failure */
+static int tok_int(const char *s)
+{
+    char *end;
+    long v = strtol(s, &end, 10);
+    return end == s ? 0 : (int)v;
+}
+
+/* stringstream >> double: prefix parse, 0 on failure */
+static double tok_double(const char *s)
+{
+    char *end;
+    double v = strtod(s, &end);
+    return end == s ? 0.0 : v;
+}
+
+/* Reaction::ReactionRead (src/Reaction.cpp:156-329) */
+static int read_mechanism(or_state *s, const char *mech, const char *thermo, char *err, int errlen)
+{
+    or_lines L;
+    if (load_lines(mech, &L)) {
+        snprintf(err, errlen, "cannot open mechanism %s", mech);
+        return -1;
+    }
+    int line = 2, ok = 1;
+    {
+        int a[100] = {0}, i = 0;
+        char *row = strdup(line < L.nlines ? L.lines[line] : "");
+        char *save = row;
+        /* split on '\t' (std::getline(stream, item, '\t')) */
+        char *q = row;
+        for (;;) {
+            char *tab = strchr(q, '\t');
+            if (tab)
+                *tab = 0;
+            if (i < 100)
+                a[i++] = item_stoi(q, &ok);
+            if (!tab)
+                break;
+            q = tab + 1;
+        }
+        free(save);
+        s->ns = a[0];
+        s->nr = a[1];
+        s->tb = a[2];
+    }
+    if (!ok || s->ns < 1 || s->nr < 1 || s->ns > OR_MAX_NS || s->nr > OR_MAX_NR) {
+        snprintf(err, errlen, "bad NS/NR in %s", mech);
+        free_lines(&L);
+        return -1;
+    }
+    int ns = s->ns, nr = s->nr;
+    s->stoi_f = (double *)calloc((size_t)ns * nr, sizeof(double));
+    s->stoi_b = (double *)calloc((size_t)ns * nr, sizeof(double));
+    s->react_tb = (double *)calloc((size_t)ns * nr, sizeof(double));
+    s->af = (double *)calloc(nr, sizeof(double));
+    s->bf = (double *)calloc(nr, sizeof(double));
+    s->eaf = (double *)calloc(nr, sizeof(double));
+    s->coeff0 = (double *)calloc(9 * (size_t)ns, sizeof(double));
+    s->coeff1 = (double *)calloc(9 * (size_t)ns, sizeof(double));
+    s->mw = (double *)calloc(ns, sizeof(double));
+    s->ri = (double *)calloc(ns, sizeof(double));
+
+    /* stoichiometric rows, forward then backward per reaction */
+    line = line + 6;
+    int cur = line;
+    for (int j = 0; j < nr; j++) {
+        for (int side = 0; side < 2; side++) {
+            char *row = strdup(cur < L.nlines ? L.lines[cur] : "");
+            cur++;
+            char *q = row;
+            int i = 0;
+            for (;;) {
+                char *tab = strchr(q, '\t');
+                if (tab)
+                    *tab = 0;
+                if (i < ns) {
+                    int v = item_stoi(q, &ok);
+                    if (side == 0)
+                        s->stoi_f[(size_t)i * nr + j] = v;
+                    else
+                        s->stoi_b[(size_t)i * nr + j] = v;
+                }
+                i++;
+                if (!tab)
+                    break;
+                q = tab + 1;
+            }
+            free(row);
+        }
+    }
+    if (!ok) {
+        snprintf(err, errlen, "bad stoichiometry row in %s", mech);
+        free_lines(&L);
+        return -1;
+    }
+    /* Arrhenius parameters: 4 tokens per reaction */
+    line = line + 2 * nr + 2;
+    or_tokstream ts;
+    char tok[512];
+    ts_seek(&ts, &L, line);
+    for (int j = 0; j < nr; j++) {
+        double b[4] = {0};
+        int i = 0;
+        while (ts_next(&ts, tok, sizeof tok)) {
+            b[i] = tok_double(tok);
+            i++;
+            if (i == 4)
+                break;
+        }
+        s->af[j] = b[1];
+        s->bf[j] = b[2];
+        s->eaf[j] = b[3];
+    }
+    /* third-body efficiencies, parsed as integers */
+    line = line + nr + 3;
+    ts_seek(&ts, &L, line);
+    {
+        int a[100] = {0};
+        for (int j = 0; j < s->tb; j++) {
+            int i = 0;
+            while (ts_next(&ts, tok, sizeof tok)) {
+                a[i] = tok_int(tok);
+                i++;
+                if (i == ns + 1)
+                    break;
+            }
+            if (a[0] < 1 || a[0] > nr) {
+                snprintf(err, errlen, "bad third-body row in %s", mech);
+                free_lines(&L);
+                return -1;
+            }
+            for (int k = 0; k < ns; k++)
+                s->react_tb[(size_t)k * nr + (a[0] - 1)] = a[k + 1];
+        }
+    }
+    /* component mole ratio */
+    line = line + s->tb + 2;
+    ts_seek(&ts, &L, line);
+    {
+        double c[100] = {0};
+        int i = 0;
+        while (ts_next(&ts, tok, sizeof tok)) {
+            c[i] = tok_double(tok);
+            i++;
+            if (i == ns)
+                break;
+        }
+        for (i = 0; i < ns; i++)
+            s->mr_temp[i] = c[i];
+    }
+    free_lines(&L);
+
+    /* thermo.dat: 19 numbers per species from line 15, 6 lines apart */
+    if (load_lines(thermo, &L)) {
+        snprintf(err, errlen, "cannot open thermo %s", thermo);
+        return -1;
+    }
+    line = 15;
+    for (int j = 0; j < ns; j++) {
+        double aa[100] = {0};
+        ts_seek(&ts, &L, line);
+        int i = 0;
+        while (ts_next(&ts, tok, sizeof tok)) {
+            aa[i] = tok_double(tok);
+            i++;
+            if (i == 19)
+                break;
+        }
+        for (int k = 0; k < 9; k++)
+            s->coeff0[(size_t)k * ns + j] = aa[k];
+        for (int k = 0; k < 9; k++)
+            s->coeff1[(size_t)k * ns + j] = aa[k + 9];
+        s->mw[j] = aa[18];
+        line = line + 6;
+    }
+    free_lines(&L);
+    for (int q = 0; q < ns; q++)
+        s->ri[q] = s->R / s->mw[q]; /* src/Reaction.cpp:427-428 */
+    return 0;
+}
+
+/* ====================================================================== */
+/* thermo (src/Reaction.cpp:1165-1220)                                    */
+/* ====================================================================== */
+
+double or_getcpi(double T, double R, int sp, const double *c0, const double *c1, int ns)
+{
+    const double *a = (T < THERMO_SWITCH_T) ? c0 : c1;
+#define A(k) a[(size_t)(k) * ns + sp]
+    return R * (A(0) * pow(T, -2) + A(1) * (1.0 / T) + A(2) + A(3) * T + A(4) * (T * T) + A(5) * pow(T, 3) +
+                A(6) * pow(T, 4));
+}
+
+double or_gethi(double T, double R, int sp, const double *c0, const double *c1, int ns)
+{
+    const double *a = (T < THERMO_SWITCH_T) ? c0 : c1;
+    return R * (-A(0) * (1.0 / T) + A(1) * log(T) + A(2) * T + A(3) * (T * T) / 2 + A(4) * pow(T, 3) / 3 +
+                A(5) * pow(T, 4) / 4 + A(6) * pow(T, 5) / 5 + A(7));
+}
+
+double or_getsi(double T, double R, int sp, const double *c0, const double *c1, int ns)
+{
+    const double *a = (T < THERMO_SWITCH_T) ? c0 : c1;
+    return R * (-A(0) * pow(T, -2) / 2 - A(1) * (1.0 / T) + A(2) * log(T) + A(3) * T + A(4) * (T * T) / 2 +
+                A(5) * pow(T, 3) / 3 + A(6) * pow(T, 4) / 4 + A(8));
+#undef A
+}
+
+/* Function::sum flag 2 (a*b) and 3 (a/b), sequential (src/Function.cpp:17-42) */
+static double sum2(const double *a, const double *b, int n)
+{
+    double r = 0;
+    for (int i = 0; i < n; i++)
+        r += (a[i] * b[i]);
+    return r;
+}
+static double sum3(const double *a, const double *b, int n)
+{
+    double r = 0;
+    for (int i = 0; i < n; i++)
+        r += (a[i] / b[i]);
+    return r;
+}
+
+/* ====================================================================== */
+/* allocation                                                             */
+/* ====================================================================== */
+
+static double *dalloc(size_t n)
+{
+    double *p = (double *)malloc(n * sizeof(double));
+    if (p)
+        for (size_t i = 0; i < n; i++)
+            p[i] = 0.0; /* the reference leaves these uninitialised; zero is one valid choice */
+    return p;
+}
+
+static void alloc_fields(or_state *s)
+{
+    size_t N = (size_t)s->NI * s->NJ * s->NK;
+    size_t ns = (size_t)s->ns, nv = ns + 4;
+    double **f1[] = {&s->U, &s->V, &s->W, &s->P, &s->D, &s->T, &s->C, &s->Ma, &s->Wav, &s->Rgas, &s->Cp,
+                     &s->H, &s->E, &s->Gamma};
+    for (size_t q = 0; q < sizeof f1 / sizeof f1[0]; q++)
+        *f1[q] = dalloc(N);
+    s->Mc = dalloc(N * ns);
+    s->Mi = dalloc(N * ns);
+    s->Yi = dalloc(N * ns);
+    s->Di = dalloc(N * ns);
+    s->CMS = dalloc(N * ns);
+    s->CS = dalloc(N * nv);
+    s->RHS = dalloc(N * nv);
+    s->F = dalloc(N * nv);
+    s->G = dalloc(N * nv);
+    s->Q = dalloc(N * nv);
+    s->CS0 = NULL;
+    s->FLR = dalloc(N * 2 * (7 + ns));
+    s->Nchem = (int *)malloc(N * sizeof(int));
+    for (size_t i = 0; i < N; i++)
+        s->Nchem[i] = 1; /* src/Reaction.cpp:102 */
+}
+
+void or_free(or_state *s)
+{
+    if (!s)
+        return;
+    double *ptrs[] = {s->xnode, s->ynode, s->znode, s->stoi_f, s->stoi_b, s->af, s->bf, s->eaf, s->react_tb,
+                      s->coeff0, s->coeff1, s->mw, s->ri, s->U, s->V, s->W, s->P, s->D, s->T, s->C, s->Ma,
+                      s->Wav, s->Rgas, s->Cp, s->H, s->E, s->Gamma, s->Mc, s->Mi, s->Yi, s->Di, s->CS,
+                      s->RHS, s->F, s->G, s->Q, s->CS0, s->CMS, s->FLR};
+    for (size_t q = 0; q < sizeof ptrs / sizeof ptrs[0]; q++)
+        free(ptrs[q]);
+    free(s->Nchem);
+    free(s);
+}
+
+int or_num_threads(int n)
+{
+#ifdef _OPENMP
+    if (n > 0)
+        omp_set_num_threads(n);
+    return omp_get_max_threads();
+#else
+    (void)n;
+    return 1;
+#endif
+}
+
+/* ====================================================================== */
+/* mesh (src/Mesh.cpp:36-142), single block                              */
+/* ====================================================================== */
+
+static void build_mesh(or_state *s)
+{
+    int bc = s->bc;
+    double dx = (s->xb - s->xa) / (s->total_ni - 1);
+    double dy = (s->yb - s->ya) / (s->total_nj - 1);
+    double dz = (s->zb - s->za) / (s->total_nk - 1);
+    s->ni = s->total_ni;
+    s->nj = s->total_nj;
+    s->nk = s->total_nk;
+    s->NI = s->ni + 2 * bc;
+    s->NJ = s->nj + 2 * bc;
+    s->NK = s->nk + 2 * bc;
+    s->xnode = dalloc(s->NI);
+    s->ynode = dalloc(s->NJ);
+    s->znode = dalloc(s->NK);
+    for (int i = bc; i < s->ni + bc; i++)
+        s->xnode[i] = s->xa + (0 * (s->ni - 1) + i - bc) * dx;
+    for (int j = bc; j < s->nj + bc; j++)
+        s->ynode[j] = s->ya + (0 * (s->nj - 1) + j - bc) * dy;
+    for (int k = bc; k < s->nk + bc; k++)
+        s->znode[k] = s->za + (0 * (s->nk - 1) + k - bc) * dz;
+    int ni = s->ni, nj = s->nj, nk = s->nk;
+    for (int i = 0; i < bc; i++) {
+        s->xnode[i] = 2 * s->xnode[bc] - s->xnode[2 * bc - i];
+        s->xnode[ni + bc + i] = 2 * s->xnode[ni + bc - 1] - s->xnode[ni + bc - 1 - i - 1];
+        s->ynode[i] = 2 * s->ynode[bc] - s->ynode[2 * bc - i];
+        s->ynode[nj + bc + i] = 2 * s->ynode[nj + bc - 1] - s->ynode[nj + bc - 1 - i - 1];
+        s->znode[i] = 2 * s->znode[bc] - s->znode[2 * bc - i];
+        s->znode[nk + bc + i] = 2 * s->znode[nk + bc - 1] - s->znode[nk + bc - 1 - i - 1];
+    }
+}
+
+/* ====================================================================== */
+/* initial conditions                                                     */
+/* ====================================================================== */
+
+/* splitmix64: counter-based generator for the seeded interface perturbation */
+static uint64_t splitmix64(uint64_t x)
+{
+    uint64_t z = x + 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+static double u01(uint64_t seed, uint64_t ctr)
+{
+    return (double)(splitmix64(seed * 0x100000001B3ull + ctr) >> 11) * (1.0 / 9007199254740992.0);
+}
+
+/* Reaction::ReactionInitial, RSBI composition (src/Reaction.cpp:412-471) */
+static void ic_rsbi_species(or_state *s)
+{
+    int ns = s->ns;
+    double cx = 0.038, cy = 0.0, Ir = 0.022, Or = 1.2 * Ir, alpha = 5;
+    for (int i = 0; i < s->NI; i++)
+        for (int j = 0; j < s->NJ; j++)
+            for (int k = 0; k < s->NK; k++) {
+                size_t id = IDX(s, i, j, k);
+                double *Y = s->Yi + id * ns, *Mi = s->Mi + id * ns, Mr[OR_MAX_NS];
+                for (int q = 0; q < ns; q++)
+                    Y[q] = Mi[q] = Mr[q] = 0.0;
+                double Mr_SUM = 0.0, Total_Mw = 0.0;
+                double ddx = s->xnode[i] - cx, ddy = s->ynode[j] - cy;
+                if (ddx * ddx + ddy * ddy > Or * Or) {
+                    Mr[4] = 1.0;
+                    Mr[8] = 3.76;
+                    for (int q = 0; q < ns; q++) {
+                        Mr_SUM += Mr[q];
+                        Total_Mw += Mr[q] * s->mw[q];
+                    }
+                    for (int q = 0; q < ns; q++) {
+                        Mi[q] = Mr[q] / Mr_SUM;
+                        Y[q] = Mr[q] * s->mw[q] / Total_Mw;
+                    }
+                } else if (ddx * ddx + ddy * ddy < Ir * Ir) {
+                    Mr[5] = 1.0;
+                    for (int q = 0; q < ns; q++) {
+                        Mr_SUM += Mr[q];
+                        Total_Mw += Mr[q] * s->mw[q];
+                    }
+                    for (int q = 0; q < ns; q++) {
+                        Mi[q] = Mr[q] / Mr_SUM;
+                        Y[q] = Mr[q] * s->mw[q] / Total_Mw;
+                    }
+                } else {
+                    double r = sqrt(ddx * ddx + ddy * ddy);
+                    double t = (r - Ir) / (Or - Ir);
+                    Y[5] = exp(-alpha * (t * t));
+                    Y[4] = (1 - Y[5]) * s->mw[4] / (s->mw[4] + 3.76 * s->mw[8]);
+                    Y[8] = (1 - Y[5]) * 3.76 * s->mw[8] / (s->mw[4] + 3.76 * s->mw[8]);
+                }
+            }
+}
+
+/* Flowfield::FieldInitial, RSBI shock (src/Flowfield.cpp:204-225) */
+static void ic_rsbi_flow(or_state *s)
+{
+    double M = s->M_wave, T_pre = s->T_pre, P_pre = s->P_pre;
+    double U_post = M * (1 - (2 + 0.40 * (M * M)) / (2.40 * (M * M))) * sqrt(1.40 * 287 * T_pre);
+    double P_post = P_pre * (1 + 2 * 1.4 / 2.4 * ((M * M) - 1.0));
+    double T_post = T_pre * (2 + 0.40 * (M * M)) / (2.40 * (M * M)) * P_post / P_pre;
+    for (int i = 0; i < s->NI; i++)
+        for (int j = 0; j < s->NJ; j++)
+            for (int k = 0; k < s->NK; k++) {
+                size_t id = IDX(s, i, j, k);
+                s->V[id] = 0.0;
+                s->W[id] = 0.0;
+                if (s->xnode[i] >= 0.01) {
+                    s->P[id] = P_pre;
+                    s->T[id] = T_pre;
+                    s->U[id] = 0.0;
+                } else {
+                    s->P[id] = P_post;
+                    s->T[id] = T_post;
+                    s->U[id] = U_post;
+                }
+            }
+}
+
+/* Spherical shock-bubble (new case "SBI"): a planar moving normal shock in
+ * N2 with the RSBI jump relations (gamma 1.4) and the N2 gas constant, hitting
+ * a stoichiometric H2-O2 bubble whose radius carries a seeded perturbation
+ * and whose interface is tanh-smoothed. */
+static void ic_sbi(or_state *s, const or_keyfile *in)
+{
+    int ns = s->ns;
+    double xr = kf_double_default(in, "bubble_xc_frac", 0.35);
+    double rr = kf_double_default(in, "bubble_r_frac", 0.2);
+    double gap = kf_double_default(in, "shock_gap_cells", 10.0);
+    double wc = kf_double_default(in, "tanh_cells", 2.0);
+    double amp = kf_double_default(in, "perturb_amp", 0.01);
+    double dx = (s->xb - s->xa) / (s->total_ni - 1);
+    double R0 = rr * (s->yb - s->ya);
+    double xc = s->xa + xr * (s->xb - s->xa);
+    double yc = 0.5 * (s->ya + s->yb), zc = 0.5 * (s->za + s->zb);
+    double xs = xc - R0 - gap * dx;
+    double width = wc * dx;
+    double nq[8][3], lq[8], phq[8];
+    for (int q = 0; q < 8; q++) {
+        double zz = 2.0 * u01(s->seed, 4 * q + 0) - 1.0;
+        double az = 6.283185307179586 * u01(s->seed, 4 * q + 1);
+        double rho = sqrt(1.0 - zz * zz);
+        nq[q][0] = rho * cos(az);
+        nq[q][1] = rho * sin(az);
+        nq[q][2] = zz;
+        lq[q] = 2.0 + (double)(splitmix64(s->seed * 31u + 7u + (uint64_t)q) % 3u);
+        phq[q] = 6.283185307179586 * u01(s->seed, 4 * q + 3);
+    }
+    double mwH2 = s->mw[5], mwO2 = s->mw[4];
+    double YH2b = 2.0 * mwH2 / (2.0 * mwH2 + mwO2);
+    double YO2b = mwO2 / (2.0 * mwH2 + mwO2);
+    double M = s->M_wave, T_pre = s->T_pre, P_pre = s->P_pre;
+    double Rn2 = s->R * 1000 / s->mw[8];
+    double U_post = M * (1 - (2 + 0.40 * (M * M)) / (2.40 * (M * M))) * sqrt(1.40 * Rn2 * T_pre);
+    double P_post = P_pre * (1 + 2 * 1.4 / 2.4 * ((M * M) - 1.0));
+    double T_post = T_pre * (2 + 0.40 * (M * M)) / (2.40 * (M * M)) * P_post / P_pre;
+    for (int i = 0; i < s->NI; i++)
+        for (int j = 0; j < s->NJ; j++)
+            for (int k = 0; k < s->NK; k++) {
+                size_t id = IDX(s, i, j, k);
+                double ex = s->xnode[i] - xc, ey = s->ynode[j] - yc, ez = s->znode[k] - zc;
+                double r = sqrt(ex * ex + ey * ey + ez * ez);
+                double p = 0.0;
+                if (r > 0.0) {
+                    for (int q = 0; q < 8; q++) {
+                        double c = (ex * nq[q][0] + ey * nq[q][1] + ez * nq[q][2]) / r;
+                        p += cos(3.141592653589793 * lq[q] * c + phq[q]);
+                    }
+                    p = p / 8.0;
+                }
+                double Rloc = R0 * (1.0 + amp * p);
+                double phi = 0.5 * (1.0 - tanh((r - Rloc) / width));
+                double *Y = s->Yi + id * ns;
+                for (int q = 0; q < ns; q++)
+                    Y[q] = 0.0;
+                Y[5] = phi * YH2b;
+                Y[4] = phi * YO2b;
+                Y[8] = 1.0 - phi;
+                s->V[id] = 0.0;
+                s->W[id] = 0.0;
+                if (s->xnode[i] < xs) {
+                    s->P[id] = P_post;
+                    s->T[id] = T_post;
+                    s->U[id] = U_post;
+                } else {
+                    s->P[id] = P_pre;
+                    s->T[id] = T_pre;
+                    s->U[id] = 0.0;
+                }
+            }
+}
+
+/* 1-D ZND initiation (new case "ZND"): stoichiometric H2-air at rest with a
+ * high-pressure, high-temperature driver of burnt products (2 H2O + 3.76 N2)
+ * over the left fraction of the domain. */
+static void ic_znd(or_state *s, const or_keyfile *in)
+{
+    int ns = s->ns;
+    double frac = kf_double_default(in, "driver_frac", 0.05);
+    double Pd = kf_double_default(in, "driver_P", 30.0 * 101325.0);
+    double Td = kf_double_default(in, "driver_T", 3000.0);
+    double xd = s->xa + frac * (s->xb - s->xa);
+    double mw_u = 2.0 * s->mw[5] + s->mw[4] + 3.76 * s->mw[8];
+    double mw_b = 2.0 * s->mw[2] + 3.76 * s->mw[8];
+    for (int i = 0; i < s->NI; i++)
+        for (int j = 0; j < s->NJ; j++)
+            for (int k = 0; k < s->NK; k++) {
+                size_t id = IDX(s, i, j, k);
+                double *Y = s->Yi + id * ns;
+                for (int q = 0; q < ns; q++)
+                    Y[q] = 0.0;
+                s->U[id] = 0.0;
+                s->V[id] = 0.0;
+                s->W[id] = 0.0;
+                if (s->xnode[i] < xd) {
+                    Y[2] = 2.0 * s->mw[2] / mw_b;
+                    Y[8] = 3.76 * s->mw[8] / mw_b;
+                    s->P[id] = Pd;
+                    s->T[id] = Td;
+                } else {
+                    Y[5] = 2.0 * s->mw[5] / mw_u;
+                    Y[4] = s->mw[4] / mw_u;
+                    Y[8] = 3.76 * s->mw[8] / mw_u;
+                    s->P[id] = s->P_pre;
+                    s->T[id] = s->T_pre;
+                }
+            }
+}
+
+/* Flowfield::FieldInitial interior tail (src/Flowfield.cpp:227-262) */
+static void field_initial_tail(or_state *s)
+{
+    int ns = s->ns, bc = s->bc;
+    double R = s->R;
+    for (int i = bc; i < s->ni + bc; i++)
+        for (int j = bc; j < s->nj + bc; j++)
+            for (int k = bc; k < s->nk + bc; k++) {
+                size_t id = IDX(s, i, j, k);
+                double Y[OR_MAX_NS], Cpi[OR_MAX_NS], Hi[OR_MAX_NS], Ei[OR_MAX_NS];
+                for (int q = 0; q < ns; q++)
+                    Y[q] = s->Yi[id * ns + q];
+                s->Wav[id] = 1.0 / sum3(Y, s->mw, ns);
+                s->Rgas[id] = R * 1000 / s->Wav[id];
+                s->D[id] = s->P[id] / (s->Rgas[id] * s->T[id]);
+                for (int q = 0; q < ns; q++) {
+                    s->Di[id * ns + q] = s->Yi[id * ns + q] * s->D[id];
+                    s->Mc[id * ns + q] = s->Di[id * ns + q] / s->mw[q] * 1000;
+                    Cpi[q] = or_getcpi(s->T[id], s->ri[q], q, s->coeff0, s->coeff1, ns);
+                    Hi[q] = or_gethi(s->T[id], s->ri[q], q, s->coeff0, s->coeff1, ns);
+                    Ei[q] = Hi[q] - s->ri[q] * s->T[id];
+                }
+                s->Cp[id] = sum2(Cpi, Y, ns) * 1000;
+                s->H[id] = sum2(Y, Hi, ns) * 1000;
+                s->E[id] = sum2(Y, Ei, ns) * 1000;
+                s->Gamma[id] = s->Cp[id] / (s->Cp[id] - R * sum3(Y, s->mw, ns) * 1000);
+                s->C[id] = sqrt(s->Gamma[id] * s->P[id] / s->D[id]);
+                double *cs = s->CS + id * (ns + 4);
+                double u = s->U[id], v = s->V[id], w = s->W[id], d = s->D[id];
+                for (int q = 0; q < ns; q++)
+                    cs[q] = d * s->Yi[id * ns + q];
+                cs[ns] = d * u;
+                cs[ns + 1] = d * v;
+                cs[ns + 2] = d * w;
+                cs[ns + 3] = d * s->E[id] + 0.5 * ((u * u) + (v * v) + (w * w)) * d;
+            }
+}
+
+static void join_path(char *out, size_t n, const char *base, const char *p)
+{
+    if (p[0] == '/' || !base || !base[0])
+        snprintf(out, n, "%s", p);
+    else
+        snprintf(out, n, "%s/%s", base, p);
+}
+
+or_state *or_init(const char *ctrl_path, const char *base_dir, char *err, int errlen)
+{
+    or_state *s = (or_state *)calloc(1, sizeof(or_state));
+    s->R = 8.31434;
+    s->Ru = 1.987;
+    s->P0 = 101325;
+    or_keyfile *ctrl = (or_keyfile *)calloc(1, sizeof(or_keyfile));
+    or_keyfile *grid = (or_keyfile *)calloc(1, sizeof(or_keyfile));
+    or_keyfile *in = (or_keyfile *)calloc(1, sizeof(or_keyfile));
+    char path[1024];
+    int ok = 1;
+    if (read_keyfile(ctrl_path, ctrl)) {
+        snprintf(err, errlen, "cannot open ctrl %s", ctrl_path);
+        goto fail;
+    }
+    /* Euler::FileRead (src/Euler.cpp:117-204) */
+    s->final_time = kf_double(ctrl, "Final-Time", &ok);
+    s->cfl = kf_double(ctrl, "cfl", &ok);
+    if (s->cfl == 0)
+        s->dt_fixed = kf_double(ctrl, "dt", &ok);
+    s->count = kf_int(ctrl, "count", &ok);
+    {
+        const char *r = kf_get(ctrl, "reaction"), *d = kf_get(ctrl, "difference"), *t = kf_get(ctrl, "time-adv");
+        if (!r || strcmp(r, "Trapezoid-Adaptive") != 0) {
+            snprintf(err, errlen, "reaction scheme must be Trapezoid-Adaptive");
+            goto fail;
+        }
+        if (!d || strcmp(d, "MUSCL_2") != 0) {
+            snprintf(err, errlen, "difference scheme must be MUSCL_2");
+            goto fail;
+        }
+        if (t && strcmp(t, "EE") == 0)
+            s->time_adv = 0;
+        else if (t && strcmp(t, "TVD_RK3") == 0)
+            s->time_adv = 1;
+        else {
+            snprintf(err, errlen, "time-adv must be EE or TVD_RK3");
+            goto fail;
+        }
+        s->diff = 1;
+        s->reaction_type = 1;
+    }
+    const char *gname = kf_get(ctrl, "gridname"), *iname = kf_get(ctrl, "initialization"),
+               *mname = kf_get(ctrl, "reaction-model"), *tname = kf_get(ctrl, "Thermofile");
+    if (!ok || !gname || !iname || !mname || !tname) {
+        snprintf(err, errlen, "ctrl file misses a key");
+        goto fail;
+    }
+    /* Mesh::MeshProcess (src/Mesh.cpp:36-84) */
+    join_path(path, sizeof path, base_dir, gname);
+    if (read_keyfile(path, grid)) {
+        snprintf(err, errlen, "cannot open grid %s", path);
+        goto fail;
+    }
+    s->total_ni = kf_int(grid, "total_ni", &ok);
+    s->xa = kf_double(grid, "xa", &ok);
+    s->xb = kf_double(grid, "xb", &ok);
+    s->total_nj = kf_int(grid, "total_nj", &ok);
+    s->ya = kf_double(grid, "ya", &ok);
+    s->yb = kf_double(grid, "yb", &ok);
+    s->total_nk = kf_int(grid, "total_nk", &ok);
+    s->za = kf_double(grid, "za", &ok);
+    s->zb = kf_double(grid, "zb", &ok);
+    s->bc = kf_int(grid, "bc", &ok);
+    if (!ok || s->total_ni < 2 || s->total_nj < 2 || s->total_nk < 2 || s->bc < 2) {
+        snprintf(err, errlen, "bad grid file (each dimension >= 2, bc >= 2)");
+        goto fail;
+    }
+    build_mesh(s);
+    /* Flowfield::InputRead (src/Flowfield.cpp:67-94) */
+    join_path(path, sizeof path, base_dir, iname);
+    if (read_keyfile(path, in)) {
+        snprintf(err, errlen, "cannot open input %s", path);
+        goto fail;
+    }
+    int NSin = kf_int(in, "NS", &ok), NRin = kf_int(in, "NR", &ok);
+    s->P_pre = kf_double(in, "P_pre", &ok);
+    s->T_pre = kf_double(in, "T_pre", &ok);
+    s->M_wave = kf_double(in, "M_wave", &ok);
+    {
+        const char *c = kf_get(in, "case");
+        snprintf(s->case_name, sizeof s->case_name, "%s", c ? c : "RSBI");
+        s->seed = (unsigned long long)kf_double_default(in, "seed", 20240601.0);
+    }
+    /* Reaction::ReactionRead */
+    char mpath[1024], tpath[1024];
+    join_path(mpath, sizeof mpath, base_dir, mname);
+    join_path(tpath, sizeof tpath, base_dir, tname);
+    if (read_mechanism(s, mpath, tpath, err, errlen))
+        goto fail;
+    if (!ok || NSin != s->ns || NRin != s->nr) {
+        snprintf(err, errlen, "input NS/NR do not match the mechanism");
+        goto fail;
+    }
+    alloc_fields(s);
+    if (strcmp(s->case_name, "RSBI") == 0) {
+        ic_rsbi_species(s);
+        ic_rsbi_flow(s);
+    } else if (strcmp(s->case_name, "SBI") == 0) {
+        ic_sbi(s, in);
+    } else if (strcmp(s->case_name, "ZND") == 0) {
+        ic_znd(s, in);
+    } else {
+        snprintf(err, errlen, "unknown case %s", s->case_name);
+        goto fail;
+    }
+    field_initial_tail(s);
+    s->dt = 0.0;
+    s->time = 0.0;
+    free(ctrl);
+    free(grid);
+    free(in);
+    return s;
+fail:
+    free(ctrl);
+    free(grid);
+    free(in);
+    or_free(s);
+    return NULL;
+}
+
+/* ====================================================================== */
+/* CFL (src/Flowfield.cpp:276-306)                                        */
+/* ====================================================================== */
+
+double or_cfl(or_state *s, double cfl, double final_time)
+{
+    double dx = s->xnode[1] - s->xnode[0];
+    double dy = s->ynode[1] - s->ynode[0];
+    double dt_conv = 1e10, temp;
+    int bc = s->bc;
+    for (int i = bc; i < s->ni + bc; i++)
+        for (int j = bc; j < s->nj + bc; j++)
+            for (int k = bc; k < s->nk + bc; k++) {
+                size_t id = IDX(s, i, j, k);
+                double a = dx / (fabs(s->U[id]) + s->C[id]);
+                double b = dy / (fabs(s->V[id]) + s->C[id]);
+                temp = (b < a) ? b : a; /* std::min */
+                if (temp < dt_conv)
+                    dt_conv = temp;
+            }
+    double dt = cfl * dt_conv;
+    if (s->time + dt > final_time)
+        dt = final_time - s->time;
+    return dt;
+}
+
+/* ====================================================================== */
+/* ghost fill (src/Flowfield.cpp:593-807)                                 */
+/* ====================================================================== */
+
+static void copy_ghost(or_state *s, size_t dst, size_t src, int flip_v, int flip_w, int full)
+{
+    int ns = s->ns;
+    s->U[dst] = s->U[src];
+    s->V[dst] = flip_v ? -s->V[src] : s->V[src];
+    s->W[dst] = flip_w ? -s->W[src] : s->W[src];
+    s->P[dst] = s->P[src];
+    if (full) {
+        s->D[dst] = s->D[src];
+        s->T[dst] = s->T[src];
+        s->H[dst] = s->H[src];
+        s->Gamma[dst] = s->Gamma[src];
+        s->E[dst] = s->E[src];
+    }
+    for (int q = 0; q < ns; q++)
+        s->Yi[dst * ns + q] = s->Yi[src * ns + q];
+}
+
+void or_field_boundary(or_state *s)
+{
+    int bc = s->bc, ni = s->ni, nj = s->nj, nk = s->nk, ns = s->ns;
+    /* left: freestream (mirror copy) */
+    {
+        const int xl = 2 * bc - 1;
+        for (int i = 0; i < bc; i++)
+            for (int j = bc; j < nj + bc; j++)
+                for (int k = bc; k < nk + bc; k++)
+                    copy_ghost(s, IDX(s, i, j, k), IDX(s, xl - i, j, k), 0, 0, 1);
+    }
+    /* right: outflow */
+    {
+        const int xr = 2 * (ni + bc) - 1;
+        for (int i = ni + bc; i < ni + 2 * bc; i++)
+            for (int j = bc; j < nj + bc; j++)
+                for (int k = bc; k < nk + bc; k++)
+                    copy_ghost(s, IDX(s, i, j, k), IDX(s, xr - i, j, k), 0, 0, 1);
+    }
+    /* front: symmetry, v -> -v, thermo recomputed from T, Y, P */
+    {
+        const int yf = 2 * bc - 1;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static)
+#endif
+        for (int i = 0; i < ni + 2 * bc; i++)
+            for (int j = 0; j < bc; j++)
+                for (int k = bc; k < nk + bc; k++) {
+                    size_t d = IDX(s, i, j, k), o = IDX(s, i, yf - j, k);
+                    double Y[OR_MAX_NS], Cpi[OR_MAX_NS], Hi[OR_MAX_NS], Ei[OR_MAX_NS];
+                    s->U[d] = s->U[o];
+                    s->V[d] = -s->V[o];
+                    s->W[d] = s->W[o];
+                    s->P[d] = s->P[o];
+                    s->T[d] = s->T[o];
+                    for (int q = 0; q < ns; q++) {
+                        s->Yi[d * ns + q] = s->Yi[o * ns + q];
+                        Y[q] = s->Yi[d * ns + q];
+                        s->Mc[d * ns + q] = s->Mc[o * ns + q];
+                        Cpi[q] = or_getcpi(s->T[d], s->ri[q], q, s->coeff0, s->coeff1, ns);
+                        Hi[q] = or_gethi(s->T[d], s->ri[q], q, s->coeff0, s->coeff1, ns);
+                        Ei[q] = Hi[q] - s->ri[q] * s->T[d];
+                    }
+                    s->Rgas[d] = sum2(Y, s->ri, ns) * 1000;
+                    s->D[d] = s->P[d] / (s->T[d] * s->Rgas[d]);
+                    s->Cp[d] = sum2(Cpi, Y, ns) * 1000;
+                    s->H[d] = sum2(Y, Hi, ns) * 1000;
+                    s->E[d] = sum2(Y, Ei, ns) * 1000;
+                    s->Gamma[d] = s->Cp[d] / (s->Cp[d] - s->Rgas[d]);
+                }
+    }
+    /* back: outflow */
+    {
+        const int yb = 2 * (nj + bc) - 1;
+        for (int i = 0; i < ni + 2 * bc; i++)
+            for (int j = nj + bc; j < nj + 2 * bc; j++)
+                for (int k = bc; k < nk + bc; k++)
+                    copy_ghost(s, IDX(s, i, j, k), IDX(s, i, yb - j, k), 0, 0, 1);
+    }
+    /* down / up: w -> -w */
+    {
+        const int zd = 2 * bc - 1, zu = 2 * (nk + bc) - 1;
+        for (int i = 0; i < ni + 2 * bc; i++)
+            for (int j = 0; j < nj + 2 * bc; j++)
+                for (int k = 0; k < bc; k++)
+                    copy_ghost(s, IDX(s, i, j, k), IDX(s, i, j, zd - k), 0, 1, 1);
+        for (int i = 0; i < ni + 2 * bc; i++)
+            for (int j = 0; j < nj + 2 * bc; j++)
+                for (int k = nk + bc; k < nk + 2 * bc; k++)
+                    copy_ghost(s, IDX(s, i, j, k), IDX(s, i, j, zu - k), 0, 1, 1);
+    }
+}
+
+/* ====================================================================== */
+/* MUSCL_2 with Minmod (include/Differ.hpp:33-43, 149-238)                */
+/* ====================================================================== */
+
+double or_minmod(double R)
+{
+    double m = 0.0;
+    if (R > 0)
+        m = (1.0 < R) ? 1.0 : R; /* std::min(R, 1.0) */
+    else
+        m = 0.0;
+    return m;
+}
+
+/* FLR has 2 entries per point: (i,j,k,0) left state, (i,j,k,1) right state
+ * of the face between point (i,j,k) and its + neighbour in `direction`. */
+void or_muscl2(int direction, int NI, int NJ, int NK, int bc, const double *fi, double *FLR)
+{
+    int ni = NI - 2 * bc, nj = NJ - 2 * bc, nk = NK - 2 * bc;
+    double kk = 1.0 / 3.0;
+    size_t si = (size_t)NJ * NK, sj = (size_t)NK, sk = 1;
+    size_t st = direction == 1 ? si : (direction == 2 ? sj : sk);
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static)
+#endif
+    for (int i = bc - 1; i < ni + bc; i++)
+        for (int j = bc - 1; j < nj + bc; j++)
+            for (int k = bc - 1; k < nk + bc; k++) {
+                size_t id = i * si + j * sj + k * sk;
+                double A0 = fi[id] - fi[id - st];
+                double A1 = fi[id + st] - fi[id];
+                double A2 = fi[id + 2 * st] - fi[id + st];
+                double L = A1 / (A0 + 1e-12);
+                double R = A1 / (A2 + 1e-12);
+                double L1 = or_minmod(L), L2 = or_minmod(1.0 / L);
+                double R1 = or_minmod(R), R2 = or_minmod(1.0 / R);
+                FLR[id * 2 + 0] = fi[id] + 0.25 * ((1.0 - kk) * L1 + (1 + kk) * L2 * L) * A0;
+                FLR[id * 2 + 1] = fi[id + st] - 0.25 * ((1.0 - kk) * R1 + (1 + kk) * R2 * R) * A2;
+            }
+}
+
+/* ====================================================================== */
+/* AUSM (src/Flowfield.cpp:1757-1931)                                     */
+/* ====================================================================== */
+
+/* L/R layout: [P, D, U, V, W, H, Gamma, Y0..Yns-1] */
+void or_ausm_face(int direction, const double *Lf, const double *Rf, int ns, double *Fi)
+{
+    double PL = Lf[0], PR = Rf[0], DL = Lf[1], DR = Rf[1], UL = Lf[2], UR = Rf[2], VL = Lf[3], VR = Rf[3],
+           WL = Lf[4], WR = Rf[4], HL = Lf[5], HR = Rf[5], GL = Lf[6], GR = Rf[6];
+    const double *YL = Lf + 7, *YR = Rf + 7;
+    double CL = sqrt(GL * PL / DL);
+    double CR = sqrt(GR * PR / DR);
+    double CI = 0.5 * (CL + CR);
+    double MaL, MaR, B1, B2, G1, G2;
+    if (direction == 1) {
+        MaL = UL / CI;
+        MaR = UR / CI;
+    } else if (direction == 2) {
+        MaL = VL / CI;
+        MaR = VR / CI;
+    } else {
+        MaL = WL / CI;
+        MaR = WR / CI;
+    }
+    if (fabs(MaL) < 1.0) {
+        double a = MaL + 1.0, b = (MaL * MaL) - 1.0;
+        B1 = 0.25 * (a * a) + 0.125 * (b * b);
+        G1 = 0.25 * (a * a) * (2.0 - MaL) + 3.0 / 16.0 * MaL * (b * b);
+    } else {
+        B1 = 0.5 * (MaL + fabs(MaL));
+        G1 = B1 / MaL;
+    }
+    if (fabs(MaR) < 1.0) {
+        double a = MaR - 1.0, b = (MaR * MaR) - 1.0;
+        B2 = -0.25 * (a * a) - 0.125 * (b * b);
+        G2 = 0.25 * (a * a) * (2.0 + MaR) - 3.0 / 16.0 * MaR * (b * b);
+    } else {
+        B2 = 0.5 * (MaR - fabs(MaR));
+        G2 = B2 / MaR;
+    }
+    double MaI = B1 + B2;
+    double VI = MaI * CI;
+    int m = ns + direction - 1; /* momentum component carrying the pressure term */
+    if (VI >= 0.0) {
+        for (int q = 0; q < ns; q++)
+            Fi[q] = VI * DL * YL[q];
+        Fi[ns] = VI * DL * UL;
+        Fi[ns + 1] = VI * DL * VL;
+        Fi[ns + 2] = VI * DL * WL;
+        Fi[m] = Fi[m] + G1 * PL + G2 * PR;
+        Fi[ns + 3] = VI * (DL * HL + 0.5 * DL * ((UL * UL) + (VL * VL) + (WL * WL)));
+    } else {
+        for (int q = 0; q < ns; q++)
+            Fi[q] = VI * DR * YR[q];
+        Fi[ns] = VI * DR * UR;
+        Fi[ns + 1] = VI * DR * VR;
+        Fi[ns + 2] = VI * DR * WR;
+        Fi[m] = Fi[m] + G1 * PL + G2 * PR;
+        Fi[ns + 3] = VI * (DR * HR + 0.5 * DR * ((UR * UR) + (VR * VR) + (WR * WR)));
+    }
+}
+
+static void ausm_dir(or_state *s, int direction, double *Fi)
+{
+    int ns = s->ns, nv = ns + 4, bc = s->bc;
+    int nf = 7 + ns;
+    size_t N = (size_t)s->NI * s->NJ * s->NK;
+    double *fields[7] = {s->P, s->D, s->U, s->V, s->W, s->H, s->Gamma};
+    for (int f = 0; f < 7; f++)
+        or_muscl2(direction, s->NI, s->NJ, s->NK, bc, fields[f], s->FLR + (size_t)f * 2 * N);
+    double *tmp = dalloc(N);
+    for (int q = 0; q < ns; q++) {
+        for (size_t id = 0; id < N; id++)
+            tmp[id] = s->Yi[id * ns + q];
+        or_muscl2(direction, s->NI, s->NJ, s->NK, bc, tmp, s->FLR + (size_t)(7 + q) * 2 * N);
+    }
+    free(tmp);
+    /* F.Fill(0) (src/Flowfield.cpp:1159-1163) */
+    for (size_t id = 0; id < N * nv; id++)
+        Fi[id] = 0.0;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static)
+#endif
+    for (int i = bc - 1; i < s->ni + bc; i++)
+        for (int j = bc - 1; j < s->nj + bc; j++)
+            for (int k = bc - 1; k < s->nk + bc; k++) {
+                size_t id = IDX(s, i, j, k);
+                double Lf[7 + OR_MAX_NS], Rf[7 + OR_MAX_NS];
+                for (int f = 0; f < nf; f++) {
+                    Lf[f] = s->FLR[(size_t)f * 2 * N + id * 2 + 0];
+                    Rf[f] = s->FLR[(size_t)f * 2 * N + id * 2 + 1];
+                }
+                or_ausm_face(direction, Lf, Rf, ns, Fi + id * nv);
+            }
+}
+
+/* ====================================================================== */
+/* TimeAdv::EE (src/TimeAdv.cpp:42-56)                                    */
+/* ====================================================================== */
+
+static void ee(or_state *s)
+{
+    int ns = s->ns, nv = ns + 4, bc = s->bc;
+    double dt = s->dt;
+    size_t si = (size_t)s->NJ * s->NK * nv, sj = (size_t)s->NK * nv, sk = nv;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static)
+#endif
+    for (int i = bc; i < s->ni + bc; i++)
+        for (int j = bc; j < s->nj + bc; j++)
+            for (int k = bc; k < s->nk + bc; k++)
+                for (int q = 0; q < nv; q++) {
+                    size_t id = IDX(s, i, j, k) * nv + q;
+                    s->RHS[id] = -dt / (s->xnode[i] - s->xnode[i - 1]) * (s->F[id] - s->F[id - si]) -
+                                 dt / (s->ynode[j] - s->ynode[j - 1]) * (s->G[id] - s->G[id - sj]) -
+                                 dt / (s->znode[k] - s->znode[k - 1]) * (s->Q[id] - s->Q[id - sk]);
+                }
+}
+
+void or_advection(or_state *s)
+{
+    ausm_dir(s, 1, s->F);
+    ausm_dir(s, 2, s->G);
+    ausm_dir(s, 3, s->Q);
+    ee(s);
+}
+
+/* ====================================================================== */
+/* state recovery (src/Flowfield.cpp:1579-1617, 2072-2131)                */
+/* ====================================================================== */
+
+double or_get_temp(const or_state *s, double T, double E, const double *Y)
+{
+    int ns = s->ns;
+    double T0 = T, T_temp = T, temp1, temp2, Cpi0[OR_MAX_NS], Hi0[OR_MAX_NS];
+    int count = 0;
+    while (count < 10) {
+        T0 = T_temp;
+        for (int q = 0; q < ns; q++) {
+            Cpi0[q] = or_getcpi(T0, s->ri[q], q, s->coeff0, s->coeff1, ns);
+            Hi0[q] = or_gethi(T0, s->ri[q], q, s->coeff0, s->coeff1, ns);
+        }
+        temp1 = (sum2(Y, Hi0, ns) - E * 1e-3) - sum2(Y, s->ri, ns) * T0;
+        temp2 = sum2(Y, Cpi0, ns) - sum2(Y, s->ri, ns);
+        T_temp = T0 - temp1 / temp2;
+        if (fabs(T_temp - T0) < 1e-6)
+            break;
+        else
+            count = count + 1;
+    }
+    return T_temp;
+}
+
+static void update_after_cs_cell(or_state *s, size_t id)
+{
+    int ns = s->ns;
+    double R = s->R;
+    const double *cs = s->CS + id * (ns + 4);
+    double Y[OR_MAX_NS], Cpi1[OR_MAX_NS];
+    double D = 0;
+    for (int q = 0; q < ns; q++) {
+        D += cs[q];
+        s->Di[id * ns + q] = cs[q];
+        s->Mc[id * ns + q] = s->Di[id * ns + q] / s->mw[q] * 1000;
+    }
+    s->D[id] = D;
+    for (int q = 0; q < ns; q++) {
+        s->Yi[id * ns + q] = cs[q] / D;
+        Y[q] = s->Yi[id * ns + q];
+    }
+    double T = or_get_temp(s, s->T[id], s->E[id], Y);
+    s->T[id] = T;
+    double U = cs[ns] / D, V = cs[ns + 1] / D, W = cs[ns + 2] / D;
+    s->U[id] = U;
+    s->V[id] = V;
+    s->W[id] = W;
+    double E = (cs[ns + 3] - 0.5 * D * ((U * U) + (V * V) + (W * W))) / D;
+    s->E[id] = E;
+    s->Wav[id] = 1.0 / sum3(Y, s->mw, ns);
+    s->Rgas[id] = R * 1000 / s->Wav[id];
+    s->P[id] = T * D * s->Rgas[id];
+    s->H[id] = s->P[id] / D + E;
+    for (int q = 0; q < ns; q++)
+        Cpi1[q] = or_getcpi(T, s->ri[q], q, s->coeff0, s->coeff1, ns);
+    s->Cp[id] = sum2(Cpi1, Y, ns) * 1000;
+    s->Gamma[id] = s->Cp[id] / (s->Cp[id] - R * sum3(Y, s->mw, ns) * 1000);
+    s->C[id] = sqrt(s->Gamma[id] * s->P[id] / D);
+    s->Ma[id] = sqrt((U * U) + (V * V) + (W * W)) / s->C[id];
+}
+
+void or_update_after_cs(or_state *s)
+{
+    int bc = s->bc;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static)
+#endif
+    for (int i = bc; i < s->ni + bc; i++)
+        for (int j = bc; j < s->nj + bc; j++)
+            for (int k = bc; k < s->nk + bc; k++)
+                update_after_cs_cell(s, IDX(s, i, j, k));
+}
+
+/* Flowfield::Update_after_Adv CPU branch (src/Flowfield.cpp:1349-1354, 1489) */
+void or_update_after_adv(or_state *s)
+{
+    int nv = s->ns + 4, bc = s->bc;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static)
+#endif
+    for (int i = bc; i < s->ni + bc; i++)
+        for (int j = bc; j < s->nj + bc; j++)
+            for (int k = bc; k < s->nk + bc; k++)
+                for (int q = 0; q < nv; q++) {
+                    size_t id = IDX(s, i, j, k) * nv + q;
+                    s->CS[id] = s->CS[id] + s->RHS[id];
+                }
+    or_update_after_cs(s);
+}
+
+/* ====================================================================== */
+/* chemistry (src/Reaction.cpp:562-942, src/Euler.cpp:461-493)            */
+/* ====================================================================== */
+
+/* Rates at (T, Mc) with Mc in mol/cm^3 (Reaction::Trapezoid:603-631 and
+ * TrapezoidPrediction:875-911).  wi = mass production rate per species. */
+void or_rates(const or_state *s, double T, const double *Mc, double *KF, double *KB, double *RR_F,
+              double *RR_B, double *R_TB, double *RR, double *Wi)
+{
+    int ns = s->ns, nr = s->nr;
+    double R = s->R, Ru = s->Ru, P0 = s->P0;
+    double Gi[OR_MAX_NS];
+    for (int q = 0; q < ns; q++) {
+        double Hi = or_gethi(T, R, q, s->coeff0, s->coeff1, ns);
+        double Si = or_getsi(T, R, q, s->coeff0, s->coeff1, ns);
+        Gi[q] = Hi - Si * T;
+    }
+    for (int r = 0; r < nr; r++) {
+        RR_F[r] = 1.0;
+        RR_B[r] = 1.0;
+        R_TB[r] = 0.0;
+    }
+    for (int r = 0; r < nr; r++) {
+        double Xr = 0.0, delta_g = 0.0;
+        KF[r] = s->af[r] * pow(T, s->bf[r]) * exp(-s->eaf[r] / (Ru * T));
+        for (int q = 0; q < ns; q++) {
+            double nb = s->stoi_b[(size_t)q * nr + r], nf = s->stoi_f[(size_t)q * nr + r];
+            Xr += (nb - nf);
+            delta_g += (nb - nf) * Gi[q];
+            RR_F[r] *= pow(Mc[q], nf);
+            RR_B[r] *= pow(Mc[q], nb);
+            R_TB[r] += Mc[q] * s->react_tb[(size_t)q * nr + r];
+        }
+        double Kp = exp(-delta_g / (R * T)) * pow((P0 * 1e-6), Xr);
+        double Kc = Kp * pow(R * T, -Xr);
+        KB[r] = KF[r] / Kc;
+        if (R_TB[r] == 0.0)
+            R_TB[r] = 1.0;
+        RR[r] = KF[r] * RR_F[r] - KB[r] * RR_B[r];
+    }
+    for (int q = 0; q < ns; q++) {
+        double temp = 0.0;
+        for (int r = 0; r < nr; r++)
+            temp += (s->stoi_b[(size_t)q * nr + r] - s->stoi_f[(size_t)q * nr + r]) * R_TB[r] * RR[r];
+        Wi[q] = s->mw[q] * temp;
+    }
+}
+
+/* TrapezoidPrediction per cell (src/Reaction.cpp:865-942).  Mc_si, Di_si are
+ * the flow field's SI arrays (mol/m^3, kg/m^3).  Returns Nchem (<= 0 flags
+ * the reference's "wrong chem" abort). */
+int or_predict_cell(const or_state *s, double T, const double *Mc_si, const double *Di_si, const double *Yi,
+                    double dtm)
+{
+    int ns = s->ns;
+    double Mc[OR_MAX_NS] = {0}, Di[OR_MAX_NS] = {0}, KF[OR_MAX_NR], KB[OR_MAX_NR], RRF[OR_MAX_NR], RRB[OR_MAX_NR],
+        RTB[OR_MAX_NR], RR[OR_MAX_NR], Wi[OR_MAX_NS];
+    for (int q = 0; q < ns; q++) {
+        Di[q] = Di_si[q] * 1e-3;
+        Mc[q] = Mc_si[q] * 1e-6;
+    }
+    or_rates(s, T, Mc, KF, KB, RRF, RRB, RTB, RR, Wi);
+    double dt = dtm, dt_temp;
+    for (int q = 0; q < ns; q++)
+        if (Yi[q] >= 1e-6)
+            if (Wi[q] != 0) {
+                dt_temp = fabs(-Di[q] / Wi[q]);
+                if (dt_temp < dt)
+                    dt = dt_temp;
+            }
+    double n = ceil(dtm / dt);
+    if (!(n >= 1.0) || n > 2147483647.0)
+        return 0;
+    return (int)n;
+}
+
+/* One adaptive trapezoid sub-step (src/Reaction.cpp:562-805): Di_si is
+ * updated in place; cms receives Wi*1e3. */
+void or_trapezoid_cell(const or_state *s, double T, const double *Mc_si, double *Di_si, double dt,
+                       double *cms)
+{
+    int ns = s->ns, nr = s->nr;
+    double Mc[OR_MAX_NS] = {0}, Di[OR_MAX_NS] = {0}, KF[OR_MAX_NR], KB[OR_MAX_NR], RRF[OR_MAX_NR], RRB[OR_MAX_NR],
+        RTB[OR_MAX_NR], RR[OR_MAX_NR], Wi[OR_MAX_NS];
+    for (int q = 0; q < ns; q++) {
+        Di[q] = Di_si[q] * 1e-3;
+        Mc[q] = Mc_si[q] * 1e-6;
+    }
+    or_rates(s, T, Mc, KF, KB, RRF, RRB, RTB, RR, Wi);
+    for (int q = 0; q < ns; q++)
+        cms[q] = Wi[q] * 1e3;
+    for (int q = 0; q < ns; q++) {
+        double Pq = 0.0, Qq = 0.0;
+        for (int r = 0; r < nr; r++) {
+            double nb = s->stoi_b[(size_t)q * nr + r], nf = s->stoi_f[(size_t)q * nr + r];
+            Qq += (nb * KF[r] * RRF[r] + nf * KB[r] * RRB[r]) * RTB[r];
+            Pq += (nb * KB[r] * RRB[r] + nf * KF[r] * RRF[r]) * RTB[r];
+        }
+        Qq = s->mw[q] * Qq;
+        if (Mc[q] == 0)
+            Pq = 0.0;
+        else
+            Pq = Pq / Mc[q];
+        Di_si[q] = ((1.0 - dt / 2.0 * Pq) * Di[q] + dt * Qq) / (1.0 + dt / 2.0 * Pq) * 1e3;
+    }
+}
+
+/* Flowfield::Explicit(step,i,j,k) (src/Flowfield.cpp:1522-1536) + Update_after_CS(step) */
+static void explicit_cell(or_state *s, size_t id)
+{
+    int ns = s->ns;
+    double *cs = s->CS + id * (ns + 4);
+    double D = 0.0;
+    for (int q = 0; q < ns; q++) {
+        cs[q] = s->Di[id * ns + q];
+        D += cs[q];
+    }
+    s->D[id] = D;
+    double U = s->U[id], V = s->V[id], W = s->W[id];
+    cs[ns] = U * D;
+    cs[ns + 1] = V * D;
+    cs[ns + 2] = W * D;
+    cs[ns + 3] = D * s->E[id] + 0.5 * ((U * U) + (V * V) + (W * W)) * D;
+    update_after_cs_cell(s, id);
+}
+
+void or_prediction(or_state *s)
+{
+    int ns = s->ns, bc = s->bc;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 1)
+#endif
+    for (int i = bc; i < s->ni + bc; i++)
+        for (int j = bc; j < s->nj + bc; j++)
+            for (int k = bc; k < s->nk + bc; k++) {
+                size_t id = IDX(s, i, j, k);
+                s->Nchem[id] =
+                    or_predict_cell(s, s->T[id], s->Mc + id * ns, s->Di + id * ns, s->Yi + id * ns, s->dt);
+            }
+}
+
+/* Euler::Trapezoid(type=1), CPU single-process branch (src/Euler.cpp:461-493) */
+void or_chemistry(or_state *s)
+{
+    int ns = s->ns, bc = s->bc;
+    or_prediction(s);
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 1)
+#endif
+    for (int i = bc; i < s->ni + bc; i++)
+        for (int j = bc; j < s->nj + bc; j++)
+            for (int k = bc; k < s->nk + bc; k++) {
+                size_t id = IDX(s, i, j, k);
+                int n = s->Nchem[id];
+                for (int step = 0; step < n; step++) {
+                    double dt = s->dt / n;
+                    or_trapezoid_cell(s, s->T[id], s->Mc + id * ns, s->Di + id * ns, dt, s->CMS + id * ns);
+                    explicit_cell(s, id);
+                }
+            }
+}
+
+/* ====================================================================== */
+/* SSP-RK3 advection (new; TimeAdv::TVD_RK3 is a no-op in the reference,  */
+/* src/TimeAdv.cpp:210-221).  Shu-Osher form; ghost fill and state        */
+/* recovery after every stage; the lagged-E Newton recovery of            */
+/* Update_after_CS is kept at every stage.                                */
+/* ====================================================================== */
+
+void or_advance_rk3(or_state *s)
+{
+    int nv = s->ns + 4, bc = s->bc;
+    size_t N = (size_t)s->NI * s->NJ * s->NK;
+    if (!s->CS0)
+        s->CS0 = dalloc(N * nv);
+    memcpy(s->CS0, s->CS, N * nv * sizeof(double));
+    for (int stage = 0; stage < 3; stage++) {
+        or_field_boundary(s);
+        or_advection(s);
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static)
+#endif
+        for (int i = bc; i < s->ni + bc; i++)
+            for (int j = bc; j < s->nj + bc; j++)
+                for (int k = bc; k < s->nk + bc; k++)
+                    for (int q = 0; q < nv; q++) {
+                        size_t id = IDX(s, i, j, k) * nv + q;
+                        double u0 = s->CS0[id], u = s->CS[id], l = s->RHS[id];
+                        if (stage == 0)
+                            s->CS[id] = u + l;
+                        else if (stage == 1)
+                            s->CS[id] = 0.75 * u0 + 0.25 * (u + l);
+                        else
+                            s->CS[id] = (1.0 / 3.0) * u0 + (2.0 / 3.0) * (u + l);
+                    }
+        or_update_after_cs(s);
+    }
+}
+
+/* one iteration of Euler::Computing's loop body (src/Euler.cpp:322-389) */
+void or_step(or_state *s, double dt)
+{
+    s->dt = dt;
+    if (s->time_adv == 0) {
+        or_field_boundary(s);
+        or_advection(s);
+        or_update_after_adv(s);
+    } else {
+        or_advance_rk3(s);
+    }
+    or_chemistry(s);
+    s->time += dt;
+    s->iteration += 1;
+}
+
+int or_run(or_state *s, int max_steps)
+{
+    int n = 0;
+    const double tol = 1e-12 * fabs(s->final_time);
+    while (s->final_time - s->time > tol && n < max_steps) {
+        double dt = (s->cfl == 0) ? s->dt_fixed : or_cfl(s, s->cfl, s->final_time);
+        if (dt > s->final_time - s->time)
+            dt = s->final_time - s->time;
+        or_step(s, dt);
+        n++;
+    }
+    return n;
+}
+
+/* ====================================================================== */
+/* dump (src/Euler.cpp:1539-1598)                                         */
+/* ====================================================================== */
+
+int or_write_dump(const or_state *s, const char *path, int num)
+{
+    FILE *f = fopen(path, "wb");
+    if (!f)
+        return -1;
+    int ns = s->ns, bc = s->bc;
+    double dtm = s->cfl == 0 ? s->dt_fixed : s->dt;
+    fprintf(f, "EU3D_STATE_V1_LE\n");
+    fprintf(f, "%d %d %d %d %d %d %d 0 0 0 1 1 1 0 1 %d %d %.17g %.17g\n", ns, s->total_ni, s->total_nj,
+            s->total_nk, s->ni, s->nj, s->nk, s->iteration, num, s->final_time, dtm);
+    fprintf(f, "X Y Z D T U V W E Wav Rgas Cp P H Gamma C Ma");
+    const char *names[3] = {"Di", "Yi", "Mc"};
+    for (int a = 0; a < 3; a++)
+        for (int q = 0; q < ns; q++)
+            fprintf(f, " %s%d", names[a], q);
+    for (int q = 0; q < ns + 4; q++)
+        fprintf(f, " CS%d", q);
+    for (int q = 0; q < ns; q++)
+        fprintf(f, " CMS%d", q);
+    fprintf(f, " Nchem\n");
+    double row[17 + 5 * OR_MAX_NS + 4 + 1];
+    for (int i = bc; i < bc + s->ni; i++)
+        for (int j = bc; j < bc + s->nj; j++)
+            for (int k = bc; k < bc + s->nk; k++) {
+                size_t id = IDX(s, i, j, k);
+                int n = 0;
+                double base[17] = {s->xnode[i], s->ynode[j], s->znode[k], s->D[id],     s->T[id],  s->U[id],
+                                   s->V[id],    s->W[id],    s->E[id],    s->Wav[id],   s->Rgas[id], s->Cp[id],
+                                   s->P[id],    s->H[id],    s->Gamma[id], s->C[id],    s->Ma[id]};
+                for (int q = 0; q < 17; q++)
+                    row[n++] = base[q];
+                for (int q = 0; q < ns; q++)
+                    row[n++] = s->Di[id * ns + q];
+                for (int q = 0; q < ns; q++)
+                    row[n++] = s->Yi[id * ns + q];
+                for (int q = 0; q < ns; q++)
+                    row[n++] = s->Mc[id * ns + q];
+                for (int q = 0; q < ns + 4; q++)
+                    row[n++] = s->CS[id * (ns + 4) + q];
+                for (int q = 0; q < ns; q++)
+                    row[n++] = s->CMS[id * ns + q];
+                row[n++] = (double)s->Nchem[id];
+                fwrite(row, sizeof(double), (size_t)n, f);
+            }
+    return fclose(f);
+}
+
+/* ====================================================================== */
+/* accessors                                                              */
+/* ====================================================================== */
+
+double *or_field(or_state *s, const char *n)
+{
+    struct {
+        const char *n;
+        double *p;
+    } t[] = {{"U", s->U},       {"V", s->V},     {"W", s->W},   {"P", s->P},     {"D", s->D},
+             {"T", s->T},       {"C", s->C},     {"Ma", s->Ma}, {"Wav", s->Wav}, {"Rgas", s->Rgas},
+             {"Cp", s->Cp},     {"H", s->H},     {"E", s->E},   {"Gamma", s->Gamma},
+             {"Mc", s->Mc},     {"Mi", s->Mi},   {"Yi", s->Yi}, {"Di", s->Di},   {"CS", s->CS},
+             {"RHS", s->RHS},   {"F", s->F},     {"G", s->G},   {"Q", s->Q},     {"CMS", s->CMS},
+             {"xnode", s->xnode}, {"ynode", s->ynode}, {"znode", s->znode},
+             {"coeff0", s->coeff0}, {"coeff1", s->coeff1}, {"mw", s->mw}, {"ri", s->ri},
+             {"stoi_f", s->stoi_f}, {"stoi_b", s->stoi_b}, {"react_tb", s->react_tb},
+             {"af", s->af}, {"bf", s->bf}, {"eaf", s->eaf}};
+    for (size_t q = 0; q < sizeof t / sizeof t[0]; q++)
+        if (strcmp(t[q].n, n) == 0)
+            return t[q].p;
+    return NULL;
+}
+
+int *or_nchem(or_state *s) { return s->Nchem; }
+
+void or_dims(const or_state *s, int *out)
+{
+    out[0] = s->ni;
+    out[1] = s->nj;
+    out[2] = s->nk;
+    out[3] = s->bc;
+    out[4] = s->NI;
+    out[5] = s->NJ;
+    out[6] = s->NK;
+    out[7] = s->ns;
+}
+
+void or_info(const or_state *s, int *iout, double *dout)
+{
+    iout[0] = s->nr;
+    iout[1] = s->tb;
+    iout[2] = s->iteration;
+    iout[3] = s->time_adv;
+    dout[0] = s->dt;
+    dout[1] = s->time;
+    dout[2] = s->final_time;
+    dout[3] = s->cfl;
+    dout[4] = s->dt_fixed;
+}
+
+void or_set_dt(or_state *s, double dt) { s->dt = dt; }
+
+/* FileReader probe for tests: kind 0 int, 1 double, 2 string; -1 if absent */
+int or_read_key(const char *path, const char *key, int kind, int *ival, double *dval, char *sval, int slen)
+{
+    or_keyfile *kf = (or_keyfile *)calloc(1, sizeof(or_keyfile));
+    int ok = 1, rc = 0;
+    if (read_keyfile(path, kf) || !kf_get(kf, key))
+        rc = -1;
+    else if (kind == 0)
+        *ival = kf_int(kf, key, &ok);
+    else if (kind == 1)
+        *dval = kf_double(kf, key, &ok);
+    else
+        snprintf(sval, (size_t)slen, "%s", kf_get(kf, key));
+    free(kf);
+    return rc;
+}

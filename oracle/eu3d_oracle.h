@@ -1,0 +1,119 @@
+/*
+ * eu3d_oracle.h -- CPU restatement of the EU3D reactive-Euler step.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is the oracle that the HIP implementation
+ * is checked against.  Only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may load it; the product library never links
+ * or calls it.
+ *
+ * It restates the reference CPU path (use_cuda=false) function by function:
+ * same loop order, same sequential summation order (src/Function.cpp:17-42),
+ * same pow/exp/log call forms, built with -ffp-contract=off (Makefile:6).
+ * Each function below names the reference file:line it follows.
+ *
+ * Layout is the reference's Array<double,D>: row-major, last index fastest
+ * (include/Array.hpp:33-43).  A scalar field has NI*NJ*NK entries with
+ * NI = ni + 2*bc; a species field has NI*NJ*NK*ns entries indexed
+ * ((i*NJ + j)*NK + k)*ns + s; the conserved vector has ns+4 components.
+ */
+#ifndef EU3D_ORACLE_H
+#define EU3D_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define OR_MAX_NS 16
+#define OR_MAX_NR 24
+
+typedef struct or_state {
+    /* mesh (src/Mesh.cpp:96-142) */
+    int ni, nj, nk, bc;          /* interior nodes per direction, ghost layers */
+    int NI, NJ, NK;              /* padded extents */
+    int total_ni, total_nj, total_nk;
+    double xa, xb, ya, yb, za, zb;
+    double *xnode, *ynode, *znode;
+
+    /* mechanism + thermo (src/Reaction.cpp:156-329) */
+    int ns, nr, tb;
+    double *stoi_f, *stoi_b;     /* [ns*nr], s*nr + r */
+    double *af, *bf, *eaf;       /* [nr] */
+    double *react_tb;            /* [ns*nr] */
+    double *coeff0, *coeff1;     /* [9*ns], k*ns + s */
+    double *mw, *ri;             /* [ns] */
+    double mr_temp[OR_MAX_NS];
+    double R, Ru, P0;
+
+    /* input keys (src/Flowfield.cpp:81-91) plus the case selector */
+    double P_pre, T_pre, M_wave;
+    char case_name[64];
+    unsigned long long seed;
+
+    /* ctrl keys (src/Euler.cpp:117-204) */
+    double final_time, cfl, dt_fixed;
+    int count;
+    int time_adv;                /* 0 = EE, 1 = TVD_RK3 */
+    int diff;                    /* 1 = MUSCL_2 (only supported scheme) */
+    int reaction_type;           /* 1 = Trapezoid-Adaptive */
+
+    /* flow field (include/Flowfield.hpp:68-110) */
+    double *U, *V, *W, *P, *D, *T, *C, *Ma, *Wav, *Rgas, *Cp, *H, *E, *Gamma;
+    double *Mc, *Mi, *Yi, *Di;   /* species fields */
+    double *CS, *RHS, *F, *G, *Q;/* ns+4 components */
+    double *CS0;                 /* RK3 stage-0 copy */
+    double *CMS;                 /* last sub-step mass production rate *1e3 */
+    int *Nchem;
+    double *FLR;                 /* 16 reconstructed fields x 2 sides (scratch) */
+
+    double dt, time;
+    int iteration;
+} or_state;
+
+/* ---- construction / teardown ---------------------------------------- */
+or_state *or_init(const char *ctrl_path, const char *base_dir, char *err, int errlen);
+void or_free(or_state *s);
+int or_num_threads(int n);     /* set OpenMP threads (if built with OpenMP); returns threads in use */
+
+/* ---- the step and its pieces ---------------------------------------- */
+double or_cfl(or_state *s, double cfl, double final_time);       /* Flowfield::CFLcondition */
+void or_field_boundary(or_state *s);                              /* FieldBoundary_3dRSBI */
+void or_advection(or_state *s);                                   /* Advection(EE, MUSCL_2) -> RHS */
+void or_update_after_adv(or_state *s);                            /* CS += RHS; Update_after_CS() */
+void or_update_after_cs(or_state *s);                             /* Update_after_CS() */
+void or_chemistry(or_state *s);                                   /* Euler::Trapezoid(1), CPU branch */
+void or_prediction(or_state *s);                                  /* TrapezoidPrediction only */
+void or_advance_rk3(or_state *s);                                 /* SSP-RK3 advection (new) */
+void or_step(or_state *s, double dt);                             /* one Computing iteration with given dt */
+int or_run(or_state *s, int max_steps);                           /* Computing loop, capped */
+
+/* ---- point functions (kernel-level probes) ------------------------- */
+double or_getcpi(double T, double R, int sp, const double *coeff0, const double *coeff1, int ns);
+double or_gethi(double T, double R, int sp, const double *coeff0, const double *coeff1, int ns);
+double or_getsi(double T, double R, int sp, const double *coeff0, const double *coeff1, int ns);
+double or_minmod(double r);
+void or_muscl2(int direction, int NI, int NJ, int NK, int bc, const double *fi, double *FLR);
+void or_ausm_face(int direction, const double *L, const double *R, int ns, double *flux);
+double or_get_temp(const or_state *s, double T, double E, const double *Yi);
+void or_rates(const or_state *s, double T, const double *Mc_si, double *kf, double *kb,
+              double *rr_f, double *rr_b, double *r_tb, double *rr, double *wi);
+int or_predict_cell(const or_state *s, double T, const double *Mc_si, const double *Di_si,
+                    const double *Yi, double dtm);
+void or_trapezoid_cell(const or_state *s, double T, const double *Mc_si, double *Di_si,
+                       double dt_sub, double *cms);
+
+/* ---- EU3D_STATE_V1_LE dump (src/Euler.cpp:1539-1598) --------------- */
+int or_write_dump(const or_state *s, const char *path, int num);
+
+/* ---- field access for ctypes ---------------------------------------- */
+double *or_field(or_state *s, const char *name);
+int *or_nchem(or_state *s);
+void or_dims(const or_state *s, int *out /* [8]: ni nj nk bc NI NJ NK ns */);
+void or_info(const or_state *s, int *iout /* [4]: nr tb iteration time_adv */,
+             double *dout /* [5]: dt time final_time cfl dt_fixed */);
+void or_set_dt(or_state *s, double dt);
+int or_read_key(const char *path, const char *key, int kind, int *ival, double *dval, char *sval, int slen);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,208 @@
+"""ctypes binding of the CPU oracle (test infrastructure only).
+
+Only tests/, __graft_entry__.smoke() (as the checker) and bench.py's
+cpu_baseline leg may import this module.  The product never does.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+_SIGS = {
+    "or_init": (ctypes.c_void_p, [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int]),
+    "or_free": (None, [ctypes.c_void_p]),
+    "or_num_threads": (ctypes.c_int, [ctypes.c_int]),
+    "or_cfl": (ctypes.c_double, [ctypes.c_void_p, ctypes.c_double, ctypes.c_double]),
+    "or_field_boundary": (None, [ctypes.c_void_p]),
+    "or_advection": (None, [ctypes.c_void_p]),
+    "or_update_after_adv": (None, [ctypes.c_void_p]),
+    "or_update_after_cs": (None, [ctypes.c_void_p]),
+    "or_chemistry": (None, [ctypes.c_void_p]),
+    "or_prediction": (None, [ctypes.c_void_p]),
+    "or_advance_rk3": (None, [ctypes.c_void_p]),
+    "or_step": (None, [ctypes.c_void_p, ctypes.c_double]),
+    "or_run": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "or_getcpi": (ctypes.c_double, [ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_void_p,
+                                    ctypes.c_void_p, ctypes.c_int]),
+    "or_gethi": (ctypes.c_double, [ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_void_p,
+                                   ctypes.c_void_p, ctypes.c_int]),
+    "or_getsi": (ctypes.c_double, [ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_void_p,
+                                   ctypes.c_void_p, ctypes.c_int]),
+    "or_minmod": (ctypes.c_double, [ctypes.c_double]),
+    "or_muscl2": (None, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                         ctypes.c_void_p, ctypes.c_void_p]),
+    "or_ausm_face": (None, [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]),
+    "or_get_temp": (ctypes.c_double, [ctypes.c_void_p, ctypes.c_double, ctypes.c_double, ctypes.c_void_p]),
+    "or_rates": (None, [ctypes.c_void_p, ctypes.c_double, ctypes.c_void_p] + [ctypes.c_void_p] * 7),
+    "or_predict_cell": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, ctypes.c_void_p, ctypes.c_void_p,
+                                       ctypes.c_void_p, ctypes.c_double]),
+    "or_trapezoid_cell": (None, [ctypes.c_void_p, ctypes.c_double, ctypes.c_void_p, ctypes.c_void_p,
+                                 ctypes.c_double, ctypes.c_void_p]),
+    "or_write_dump": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int]),
+    "or_field": (ctypes.POINTER(ctypes.c_double), [ctypes.c_void_p, ctypes.c_char_p]),
+    "or_nchem": (ctypes.POINTER(ctypes.c_int), [ctypes.c_void_p]),
+    "or_dims": (None, [ctypes.c_void_p, ctypes.c_void_p]),
+    "or_info": (None, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    "or_set_dt": (None, [ctypes.c_void_p, ctypes.c_double]),
+    "or_read_key": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int, ctypes.c_void_p,
+                                   ctypes.c_void_p, ctypes.c_char_p, ctypes.c_int]),
+}
+
+_libs = {}
+
+
+def build():
+    """Compile the oracle (make, gcc)."""
+    import subprocess
+    subprocess.check_call(["make", "-s", "-C", HERE])
+
+
+def lib(omp=False):
+    name = "liboracle_omp.so" if omp else "liboracle.so"
+    if name not in _libs:
+        path = os.path.join(HERE, name)
+        if not os.path.exists(path):
+            build()
+        L = ctypes.CDLL(path)
+        for fn, (res, args) in _SIGS.items():
+            f = getattr(L, fn)
+            f.restype = res
+            f.argtypes = args
+        _libs[name] = L
+    return _libs[name]
+
+
+def ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+SCALAR_FIELDS = ["U", "V", "W", "P", "D", "T", "C", "Ma", "Wav", "Rgas", "Cp", "H", "E", "Gamma"]
+SPECIES_FIELDS = ["Mc", "Mi", "Yi", "Di", "CMS"]
+VECTOR_FIELDS = ["CS", "RHS", "F", "G", "Q"]
+
+
+class Oracle:
+    """One solver state of the CPU restatement, built from a ctrl.txt."""
+
+    def __init__(self, ctrl_path, base_dir=None, omp=False, threads=0):
+        self.L = lib(omp)
+        if omp and threads:
+            self.L.or_num_threads(threads)
+        err = ctypes.create_string_buffer(512)
+        base = (base_dir or os.path.dirname(os.path.abspath(ctrl_path))).encode()
+        self.h = self.L.or_init(ctrl_path.encode(), base, err, 512)
+        if not self.h:
+            raise RuntimeError("oracle init failed: " + err.value.decode())
+        d = np.zeros(8, dtype=np.int32)
+        self.L.or_dims(self.h, ptr(d))
+        self.ni, self.nj, self.nk, self.bc, self.NI, self.NJ, self.NK, self.ns = map(int, d)
+
+    def __del__(self):
+        h = getattr(self, "h", None)
+        if h:
+            self.L.or_free(h)
+            self.h = None
+
+    # field views (shared memory with the C state, reference layout)
+    def field(self, name):
+        p = self.L.or_field(self.h, name.encode())
+        if not p:
+            raise KeyError(name)
+        shape = (self.NI, self.NJ, self.NK)
+        if name in SCALAR_FIELDS:
+            pass
+        elif name in SPECIES_FIELDS:
+            shape = shape + (self.ns,)
+        elif name in VECTOR_FIELDS:
+            shape = shape + (self.ns + 4,)
+        elif name in ("xnode",):
+            shape = (self.NI,)
+        elif name in ("ynode",):
+            shape = (self.NJ,)
+        elif name in ("znode",):
+            shape = (self.NK,)
+        elif name in ("coeff0", "coeff1"):
+            shape = (9, self.ns)
+        elif name in ("mw", "ri"):
+            shape = (self.ns,)
+        else:
+            raise KeyError(name)
+        n = int(np.prod(shape))
+        return np.ctypeslib.as_array(p, shape=(n,)).reshape(shape)
+
+    def nchem(self):
+        p = self.L.or_nchem(self.h)
+        return np.ctypeslib.as_array(p, shape=(self.NI * self.NJ * self.NK,)).reshape(self.NI, self.NJ, self.NK)
+
+    def interior(self, a):
+        b = self.bc
+        return a[b:b + self.ni, b:b + self.nj, b:b + self.nk]
+
+    def step(self, dt):
+        self.L.or_step(self.h, dt)
+
+    def run(self, n):
+        return self.L.or_run(self.h, n)
+
+    def cfl(self, cfl, final_time):
+        return self.L.or_cfl(self.h, cfl, final_time)
+
+    def field_boundary(self):
+        self.L.or_field_boundary(self.h)
+
+    def advection(self):
+        self.L.or_advection(self.h)
+
+    def update_after_adv(self):
+        self.L.or_update_after_adv(self.h)
+
+    def update_after_cs(self):
+        self.L.or_update_after_cs(self.h)
+
+    def chemistry(self):
+        self.L.or_chemistry(self.h)
+
+    def prediction(self):
+        self.L.or_prediction(self.h)
+
+    def advance_rk3(self):
+        self.L.or_advance_rk3(self.h)
+
+    def set_dt(self, dt):
+        """or_step sets dt itself; this is for driving the pieces one by one."""
+        self.L.or_set_dt(self.h, dt)
+
+    def info(self):
+        iv = np.zeros(4, dtype=np.int32)
+        dv = np.zeros(5)
+        self.L.or_info(self.h, ptr(iv), ptr(dv))
+        return dict(nr=int(iv[0]), tb=int(iv[1]), iteration=int(iv[2]), time_adv=int(iv[3]),
+                    dt=dv[0], time=dv[1], final_time=dv[2], cfl=dv[3], dt_fixed=dv[4])
+
+    def write_dump(self, path, num=1):
+        return self.L.or_write_dump(self.h, path.encode(), num)
+
+    def get_temp(self, T, E, Y):
+        Y = np.ascontiguousarray(Y, dtype=np.float64)
+        return self.L.or_get_temp(self.h, T, E, ptr(Y))
+
+    def rates(self, T, Mc_cgs):
+        nr = len(self.field_raw("af", None))
+        out = [np.zeros(nr) for _ in range(6)] + [np.zeros(self.ns)]
+        Mc = np.ascontiguousarray(Mc_cgs, dtype=np.float64)
+        self.L.or_rates(self.h, T, ptr(Mc), *[ptr(o) for o in out])
+        return dict(zip(["kf", "kb", "rr_f", "rr_b", "r_tb", "rr", "wi"], out))
+
+    def field_raw(self, name, n):
+        p = self.L.or_field(self.h, name.encode())
+        if n is None:
+            n = {"af": self.nr, "bf": self.nr, "eaf": self.nr}[name]
+        return np.ctypeslib.as_array(p, shape=(n,))
+
+    @property
+    def nr(self):
+        return self.info()["nr"]
+
+
