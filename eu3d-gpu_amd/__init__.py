@@ -1,0 +1,234 @@
+"""MI355X (gfx950) reactive-Euler step: Python access to the C ABI.
+
+libeu3d.so (built by build.py) holds the HIP kernels and the C ABI declared
+in include/eu3d.h (device context, step entry points) and include/eu3d_host.h
+(the reference-format readers and the Euler driver).  This module only binds
+them with ctypes; there is no Python or CPU fallback for any compute entry
+point: if the library is missing, load() raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libeu3d.so")
+INPUT = os.path.join(HERE, "input")
+
+EU3D_OK = 0
+EU3D_ERR_ARG = -1
+EU3D_ERR_HIP = -2
+EU3D_ERR_STATE = -3
+EU3D_ERR_HEALTH = -4
+SCHEME_EE = 0
+SCHEME_TVD_RK3 = 1
+CHEM_COMPACT = 0x1
+CHEM_SKIP = 0x2
+CHEM_KEEP_CMS = 0x4
+
+_vp = ctypes.c_void_p
+_i = ctypes.c_int
+_d = ctypes.c_double
+
+
+class Mesh(ctypes.Structure):
+    _fields_ = [("ni", _i), ("nj", _i), ("nk", _i), ("bc", _i), ("xnode", _vp), ("ynode", _vp), ("znode", _vp)]
+
+
+class Mechanism(ctypes.Structure):
+    _fields_ = [("ns", _i), ("nr", _i), ("R", _d), ("Ru", _d), ("P0", _d), ("mw", _vp), ("ri", _vp),
+                ("coeff0", _vp), ("coeff1", _vp), ("af", _vp), ("bf", _vp), ("eaf", _vp),
+                ("stoi_f", _vp), ("stoi_b", _vp), ("react_tb", _vp)]
+
+
+STATE_FIELDS = ["U", "V", "W", "P", "D", "T", "C", "Ma", "Wav", "Rgas", "Cp", "H", "E", "Gamma",
+                "Mc", "Yi", "Di", "CMS", "CS", "Nchem"]
+
+
+class State(ctypes.Structure):
+    _fields_ = [(n, _vp) for n in STATE_FIELDS]
+
+
+class Health(ctypes.Structure):
+    _fields_ = [("invalid_nchem", ctypes.c_longlong), ("nonfinite", ctypes.c_longlong), ("max_nchem", _i),
+                ("stiff_cells", ctypes.c_longlong), ("total_substeps", ctypes.c_longlong)]
+
+
+class Timing(ctypes.Structure):
+    _fields_ = [(n, _d) for n in ["flux_ms", "recover_ms", "chem_ms", "ghost_ms", "cfl_ms", "flux_total_ms",
+                                  "recover_total_ms", "chem_total_ms", "ghost_total_ms", "cfl_total_ms"]] + \
+               [(n, ctypes.c_longlong) for n in ["flux_calls", "recover_calls", "chem_calls", "ghost_calls",
+                                                 "cfl_calls"]]
+
+
+# name -> (restype, argtypes); every symbol declared in include/eu3d.h and include/eu3d_host.h
+SIGNATURES = {
+    "eu3d_create": (_i, [ctypes.POINTER(Mesh), ctypes.POINTER(Mechanism), ctypes.POINTER(_vp)]),
+    "eu3d_destroy": (_i, [_vp]),
+    "eu3d_last_error": (ctypes.c_char_p, [_vp]),
+    "eu3d_upload_state": (_i, [_vp, ctypes.POINTER(State)]),
+    "eu3d_download_state": (_i, [_vp, ctypes.POINTER(State)]),
+    "eu3d_fill_ghosts": (_i, [_vp]),
+    "eu3d_cfl_dt": (_i, [_vp, _d, _d, _d, ctypes.POINTER(_d)]),
+    "eu3d_flux_update": (_i, [_vp, _d]),
+    "eu3d_recover_state": (_i, [_vp]),
+    "eu3d_advance_rk3": (_i, [_vp, _d]),
+    "eu3d_chemistry": (_i, [_vp, _d, _i]),
+    "eu3d_step": (_i, [_vp, _i, _d, _i]),
+    "eu3d_run": (_i, [_vp, _i, _d, _d, _d, _i, _i, ctypes.POINTER(_d), ctypes.POINTER(_i)]),
+    "eu3d_health": (_i, [_vp, ctypes.POINTER(Health)]),
+    "eu3d_synchronize": (_i, [_vp]),
+    "eu3d_timing_get": (_i, [_vp, ctypes.POINTER(Timing)]),
+    "eu3d_timing_enable": (_i, [_vp, _i]),
+    "eu3d_probe_thermo": (_i, [_vp, _vp, _i, _d, _vp, _vp, _vp]),
+    "eu3d_probe_face_flux": (_i, [_vp, _i, _vp]),
+    "eu3d_probe_rates": (_i, [_vp, _vp, _vp, _i, _vp, _vp, _vp, _vp]),
+    "eu3d_solver_open": (_i, [ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(_vp), ctypes.c_char_p, _i]),
+    "eu3d_solver_open_host": (_i, [ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(_vp), ctypes.c_char_p, _i]),
+    "eu3d_solver_attach": (_i, [_vp]),
+    "eu3d_solver_close": (_i, [_vp]),
+    "eu3d_solver_ctx": (_vp, [_vp]),
+    "eu3d_solver_dims": (_i, [_vp, _vp]),
+    "eu3d_solver_info": (_i, [_vp, _vp]),
+    "eu3d_solver_run": (_i, [_vp, _i, _i]),
+    "eu3d_solver_sync": (_i, [_vp]),
+    "eu3d_solver_upload": (_i, [_vp]),
+    "eu3d_solver_field": (ctypes.POINTER(_d), [_vp, ctypes.c_char_p]),
+    "eu3d_solver_nchem": (ctypes.POINTER(_i), [_vp]),
+    "eu3d_solver_dump": (_i, [_vp, ctypes.c_char_p, _i]),
+    "eu3d_solver_error": (ctypes.c_char_p, [_vp]),
+}
+
+_lib = None
+
+
+def load():
+    """Load libeu3d.so (raises if it was not built: there is no fallback)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with eu3d-gpu_amd/build.py "
+                               "(the HIP extension is required; there is no CPU fallback)")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def ptr(a):
+    return a.ctypes.data_as(_vp)
+
+
+class EU3DError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+def check(rc, ctx=None, what=""):
+    if rc != EU3D_OK:
+        msg = load().eu3d_last_error(ctx).decode() if ctx else ""
+        raise EU3DError(rc, f"{what}: {msg}")
+
+
+SCALARS = ["U", "V", "W", "P", "D", "T", "C", "Ma", "Wav", "Rgas", "Cp", "H", "E", "Gamma"]
+SPECIES = ["Mc", "Mi", "Yi", "Di", "CMS"]
+
+
+class Solver:
+    """The reference's Euler driver on the GPU (include/eu3d_host.h)."""
+
+    def __init__(self, ctrl, base_dir=None, device=True):
+        L = load()
+        self.L = L
+        h = _vp()
+        err = ctypes.create_string_buffer(512)
+        base = (base_dir if base_dir is not None else os.path.dirname(os.path.abspath(ctrl))).encode()
+        opener = L.eu3d_solver_open if device else L.eu3d_solver_open_host
+        rc = opener(ctrl.encode(), base, ctypes.byref(h), err, 512)
+        if rc != EU3D_OK:
+            raise EU3DError(rc, err.value.decode())
+        self.h = h
+        d = np.zeros(10, dtype=np.int32)
+        L.eu3d_solver_dims(h, ptr(d))
+        self.ni, self.nj, self.nk, self.bc, self.NI, self.NJ, self.NK, self.ns, self.nr, self.time_adv = map(int, d)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.eu3d_solver_close(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def _check(self, rc, what):
+        if rc != EU3D_OK:
+            raise EU3DError(rc, f"{what}: {self.L.eu3d_solver_error(self.h).decode()}")
+
+    @property
+    def ctx(self):
+        return self.L.eu3d_solver_ctx(self.h)
+
+    def attach(self):
+        self._check(self.L.eu3d_solver_attach(self.h), "attach")
+
+    def run(self, steps, chem_flags=CHEM_COMPACT | CHEM_KEEP_CMS):
+        self._check(self.L.eu3d_solver_run(self.h, steps, chem_flags), "run")
+
+    def sync(self):
+        self._check(self.L.eu3d_solver_sync(self.h), "sync")
+
+    def upload(self):
+        self._check(self.L.eu3d_solver_upload(self.h), "upload")
+
+    def dump(self, path, num=1):
+        self._check(self.L.eu3d_solver_dump(self.h, path.encode(), num), "dump")
+
+    def info(self):
+        v = np.zeros(6)
+        self.L.eu3d_solver_info(self.h, ptr(v))
+        return dict(final_time=v[0], cfl=v[1], dt=v[2], time=v[3], last_dt=v[4], iteration=int(v[5]))
+
+    def field(self, name):
+        """View of a host array (reference layout); call sync() first for device results."""
+        p = self.L.eu3d_solver_field(self.h, name.encode())
+        if not p:
+            raise KeyError(name)
+        g = (self.NI, self.NJ, self.NK)
+        shapes = {"xnode": (self.NI,), "ynode": (self.NJ,), "znode": (self.NK,), "coeff0": (9, self.ns),
+                  "coeff1": (9, self.ns), "mw": (self.ns,), "ri": (self.ns,), "stoi_f": (self.ns, self.nr),
+                  "stoi_b": (self.ns, self.nr), "react_tb": (self.ns, self.nr), "af": (self.nr,),
+                  "bf": (self.nr,), "eaf": (self.nr,), "CS": g + (self.ns + 4,)}
+        if name in SCALARS:
+            shape = g
+        elif name in SPECIES:
+            shape = g + (self.ns,)
+        else:
+            shape = shapes[name]
+        n = int(np.prod(shape))
+        return np.ctypeslib.as_array(p, shape=(n,)).reshape(shape)
+
+    def nchem(self):
+        p = self.L.eu3d_solver_nchem(self.h)
+        return np.ctypeslib.as_array(p, shape=(self.NI * self.NJ * self.NK,)).reshape(self.NI, self.NJ, self.NK)
+
+    def interior(self, a):
+        b = self.bc
+        return a[b:b + self.ni, b:b + self.nj, b:b + self.nk]
+
+    def health(self):
+        h = Health()
+        rc = self.L.eu3d_health(self.ctx, ctypes.byref(h))
+        return rc, h
+
+
+def state_struct(arrays):
+    """eu3d_state pointing at numpy arrays (missing names -> NULL)."""
+    s = State()
+    for n in STATE_FIELDS:
+        a = arrays.get(n)
+        setattr(s, n, ptr(a) if a is not None else None)
+    return s

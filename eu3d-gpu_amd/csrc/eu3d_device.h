@@ -1,0 +1,436 @@
+// eu3d_device.h -- device-side physics of the reactive-Euler step (gfx950).
+//
+// Shared by all kernels.  Every expression keeps the reference CPU path's
+// operand order and association (the build uses -ffp-contract=off), so a
+// kernel reproduces the CPU result up to the last-bit differences of the
+// device's exp/log/pow; divisions and square roots are IEEE-correct on gfx950.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define EU3D_THERMO_SWITCH_T 1000.0 // include/ThermoConstants.hpp:3
+
+namespace eu3d {
+
+constexpr int kMaxNS = 16;
+constexpr int kMaxNR = 24;
+
+// Mechanism + thermo tables, resident in device memory, read with uniform
+// (scalar) loads.  Layouts follow Reaction::ReactionRead (src/Reaction.cpp:156-329).
+struct DevMech {
+    int ns, nr;
+    double R, Ru, P0;
+    double mw[kMaxNS], ri[kMaxNS];
+    double c0[9 * kMaxNS], c1[9 * kMaxNS]; // k*ns + s
+    double af[kMaxNR], bf[kMaxNR], eaf[kMaxNR];
+    double nf[kMaxNS * kMaxNR], nb[kMaxNS * kMaxNR], tb[kMaxNS * kMaxNR]; // s*nr + r
+    double p0x[kMaxNR]; // pow(P0*1e-6, Xr), evaluated on the host with the C library
+    double mxr[kMaxNR]; // -Xr, Xr = sum_s (nb - nf) in species order
+};
+
+// Grid description passed by value to kernels.
+struct Grid {
+    int ni, nj, nk, bc, NI, NJ, NK;
+    long long N;        // NI*NJ*NK
+    long long sI, sJ;   // strides of i and j (k stride is 1)
+};
+
+__device__ __forceinline__ long long gidx(const Grid &g, int i, int j, int k)
+{
+    return ((long long)i * g.NJ + j) * g.NK + k;
+}
+
+// ---------------------------------------------------------------------------
+// Powers of T for the NASA-9 polynomials.  The reference calls pow(T,n); GCC
+// folds pow(T,2) to T*T and pow(T,-1) to 1.0/T, the rest are libm calls that
+// are correctly rounded in all but rare cases.  The other powers here are
+// formed in double-double and rounded once, so they are correctly rounded in
+// all but rare cases as well.
+// ---------------------------------------------------------------------------
+struct TPow {
+    double T, Tm2, Tm1, T2, T3, T4, T5, lnT;
+};
+
+__device__ __forceinline__ void dd_mul_d(double ah, double al, double b, double &rh, double &rl)
+{
+    double p = ah * b;
+    double e = __builtin_fma(ah, b, -p);
+    e = __builtin_fma(al, b, e);
+    rh = p + e;
+    rl = e - (rh - p);
+}
+
+__device__ __forceinline__ TPow tpow(double T)
+{
+    TPow t;
+    t.T = T;
+    t.Tm1 = 1.0 / T;
+    double h2 = T * T;
+    double l2 = __builtin_fma(T, T, -h2);
+    t.T2 = h2;
+    double h3, l3, h4, l4, h5, l5;
+    dd_mul_d(h2, l2, T, h3, l3);
+    t.T3 = h3;
+    dd_mul_d(h3, l3, T, h4, l4);
+    t.T4 = h4;
+    dd_mul_d(h4, l4, T, h5, l5);
+    t.T5 = h5;
+    // 1/(h2 + l2) with one correction step
+    double q = 1.0 / h2;
+    double r = __builtin_fma(-q, h2, 1.0);
+    r = r - q * l2;
+    t.Tm2 = __builtin_fma(q, r, q);
+    t.lnT = log(T);
+    (void)l5;
+    return t;
+}
+
+// Reaction::GetCpi / GetHi / GetSi (src/Reaction.cpp:1165-1220), coefficient
+// block a[k*ns] for one species (stride ns).
+__device__ __forceinline__ double getcpi(const TPow &t, double R, const double *a, int ns)
+{
+    return R * (a[0] * t.Tm2 + a[ns] * t.Tm1 + a[2 * ns] + a[3 * ns] * t.T + a[4 * ns] * t.T2 +
+                a[5 * ns] * t.T3 + a[6 * ns] * t.T4);
+}
+__device__ __forceinline__ double gethi(const TPow &t, double R, const double *a, int ns)
+{
+    return R * (-a[0] * t.Tm1 + a[ns] * t.lnT + a[2 * ns] * t.T + a[3 * ns] * t.T2 / 2 + a[4 * ns] * t.T3 / 3 +
+                a[5 * ns] * t.T4 / 4 + a[6 * ns] * t.T5 / 5 + a[7 * ns]);
+}
+__device__ __forceinline__ double getsi(const TPow &t, double R, const double *a, int ns)
+{
+    return R * (-a[0] * t.Tm2 / 2 - a[ns] * t.Tm1 + a[2 * ns] * t.lnT + a[3 * ns] * t.T + a[4 * ns] * t.T2 / 2 +
+                a[5 * ns] * t.T3 / 3 + a[6 * ns] * t.T4 / 4 + a[8 * ns]);
+}
+
+__device__ __forceinline__ const double *coeffs(const DevMech &m, double T, int s)
+{
+    return (T < EU3D_THERMO_SWITCH_T ? m.c0 : m.c1) + s;
+}
+
+// ---------------------------------------------------------------------------
+// Flowfield::Get_temp(T, i, j, k, Yi) (src/Flowfield.cpp:2072-2131): Newton on
+// sum Y h(T) - E*1e-3 - (sum Y R) T with derivative sum Y cp - sum Y R; at
+// most 10 iterations, stop when |dT| < 1e-6.  E is the previous internal energy.
+// ---------------------------------------------------------------------------
+template <int NS>
+__device__ __forceinline__ double get_temp(const DevMech &m, double T, double E, const double (&Y)[NS])
+{
+    double T0 = T, T_temp = T;
+    int count = 0;
+    double sYR = 0.0;
+#pragma unroll
+    for (int s = 0; s < NS; s++)
+        sYR += Y[s] * m.ri[s];
+    while (count < 10) {
+        T0 = T_temp;
+        TPow t = tpow(T0);
+        double sh = 0.0, scp = 0.0;
+#pragma unroll
+        for (int s = 0; s < NS; s++) {
+            const double *a = coeffs(m, T0, s);
+            double cp = getcpi(t, m.ri[s], a, NS);
+            double h = gethi(t, m.ri[s], a, NS);
+            sh += Y[s] * h;
+            scp += Y[s] * cp;
+        }
+        double temp1 = (sh - E * 1e-3) - sYR * T0;
+        double temp2 = scp - sYR;
+        T_temp = T0 - temp1 / temp2;
+        if (fabs(T_temp - T0) < 1e-6)
+            break;
+        count = count + 1;
+    }
+    return T_temp;
+}
+
+// Full recovered state of one point (Update_after_CS, src/Flowfield.cpp:1579-1617).
+template <int NS>
+struct CellState {
+    double D, T, U, V, W, E, P, H, G, C;
+    double Y[NS];
+};
+
+// Flowfield::Update_after_CS for one point: cs[NS+4] conserved, T and E the
+// previous temperature and internal energy.
+template <int NS>
+__device__ __forceinline__ void recover(const DevMech &m, const double (&cs)[NS + 4], double Told, double Eold,
+                                        CellState<NS> &o)
+{
+    const double R = m.R;
+    double D = 0;
+#pragma unroll
+    for (int s = 0; s < NS; s++)
+        D += cs[s];
+#pragma unroll
+    for (int s = 0; s < NS; s++)
+        o.Y[s] = cs[s] / D;
+    double T = get_temp<NS>(m, Told, Eold, o.Y);
+    double U = cs[NS] / D, V = cs[NS + 1] / D, W = cs[NS + 2] / D;
+    double E = (cs[NS + 3] - 0.5 * D * ((U * U) + (V * V) + (W * W))) / D;
+    double sYM = 0.0;
+#pragma unroll
+    for (int s = 0; s < NS; s++)
+        sYM += o.Y[s] / m.mw[s];
+    double Wav = 1.0 / sYM;
+    double Rgas = R * 1000 / Wav;
+    double P = T * D * Rgas;
+    double H = P / D + E;
+    TPow t = tpow(T);
+    double scp = 0.0;
+#pragma unroll
+    for (int s = 0; s < NS; s++)
+        scp += getcpi(t, m.ri[s], coeffs(m, T, s), NS) * o.Y[s];
+    double Cp = scp * 1000;
+    double G = Cp / (Cp - R * sYM * 1000);
+    double C = sqrt(G * P / D);
+    o.D = D;
+    o.T = T;
+    o.U = U;
+    o.V = V;
+    o.W = W;
+    o.E = E;
+    o.P = P;
+    o.H = H;
+    o.G = G;
+    o.C = C;
+}
+
+// ---------------------------------------------------------------------------
+// MUSCL_2 with the minmod limiter (include/Differ.hpp:33-43, 149-238).
+// lim_s returns (1-k)*phi(r) + (1+k)*phi(1/r)*r for r = A1/(Aden+1e-12),
+// phi = minmod, k = 1/3, with the same rounding as the reference: for
+// 0 < r < 1 phi(1/r) is exactly 1 and 1/r is not needed; for r <= 0 the
+// value is +0 and for NaN r it is NaN, as in the reference.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double lim_s(double A1, double Aden)
+{
+    const double kk = 1.0 / 3.0;
+    double r = A1 / (Aden + 1e-12);
+    double s;
+    if (r > 0.0) {
+        if (r < 1.0)
+            s = (1.0 - kk) * r + (1 + kk) * 1.0 * r;
+        else
+            s = (1.0 - kk) * 1.0 + (1 + kk) * (1.0 / r) * r;
+    } else {
+        s = (r != r) ? r : 0.0;
+    }
+    return s;
+}
+
+// Left state of the face between a and a+1 from f(a-1), f(a), f(a+1).
+__device__ __forceinline__ double muscl_left(double fm, double f0, double fp)
+{
+    double A0 = f0 - fm, A1 = fp - f0;
+    return f0 + 0.25 * lim_s(A1, A0) * A0;
+}
+// Right state of the face between a and a+1 from f(a), f(a+1), f(a+2).
+__device__ __forceinline__ double muscl_right(double f0, double fp, double fpp)
+{
+    double A1 = fp - f0, A2 = fpp - fp;
+    return fp - 0.25 * lim_s(A1, A2) * A2;
+}
+// The state on the upwind side only (left if up, else right); identical
+// arithmetic to muscl_left / muscl_right, selected without divergence.
+__device__ __forceinline__ double muscl_upwind(bool up, double fm, double f0, double fp, double fpp)
+{
+    double A1 = fp - f0;
+    double Aden = up ? (f0 - fm) : (fpp - fp);
+    double base = up ? f0 : fp;
+    double t = 0.25 * lim_s(A1, Aden) * Aden;
+    return up ? base + t : base - t;
+}
+
+// ---------------------------------------------------------------------------
+// AUSM+ interface flux (src/Flowfield.cpp:1797-1931).  dir 0/1/2 = x/y/z.
+// Field order in prim: P D U V W H Gamma Y0..Y(NS-1), each an N-long array.
+// a = index of the left point, st = stride of the direction.
+// ---------------------------------------------------------------------------
+template <int NS>
+__device__ __forceinline__ void face_flux(int dir, const double *__restrict__ prim, long long N, long long a,
+                                          long long st, double (&F)[NS + 4])
+{
+#define LD(f, o) prim[(long long)(f) * N + a + (o) * st]
+    double PL = muscl_left(LD(0, -1), LD(0, 0), LD(0, 1));
+    double PR = muscl_right(LD(0, 0), LD(0, 1), LD(0, 2));
+    double DL = muscl_left(LD(1, -1), LD(1, 0), LD(1, 1));
+    double DR = muscl_right(LD(1, 0), LD(1, 1), LD(1, 2));
+    double GL = muscl_left(LD(6, -1), LD(6, 0), LD(6, 1));
+    double GR = muscl_right(LD(6, 0), LD(6, 1), LD(6, 2));
+    const int fn = 2 + dir; // normal velocity field
+    double UnL = muscl_left(LD(fn, -1), LD(fn, 0), LD(fn, 1));
+    double UnR = muscl_right(LD(fn, 0), LD(fn, 1), LD(fn, 2));
+
+    double CL = sqrt(GL * PL / DL);
+    double CR = sqrt(GR * PR / DR);
+    double CI = 0.5 * (CL + CR);
+    double MaL = UnL / CI;
+    double MaR = UnR / CI;
+    double B1, B2, G1, G2;
+    if (fabs(MaL) < 1.0) {
+        double p = MaL + 1.0, q = (MaL * MaL) - 1.0;
+        B1 = 0.25 * (p * p) + 0.125 * (q * q);
+        G1 = 0.25 * (p * p) * (2.0 - MaL) + 3.0 / 16.0 * MaL * (q * q);
+    } else {
+        B1 = 0.5 * (MaL + fabs(MaL));
+        G1 = B1 / MaL;
+    }
+    if (fabs(MaR) < 1.0) {
+        double p = MaR - 1.0, q = (MaR * MaR) - 1.0;
+        B2 = -0.25 * (p * p) - 0.125 * (q * q);
+        G2 = 0.25 * (p * p) * (2.0 + MaR) - 3.0 / 16.0 * MaR * (q * q);
+    } else {
+        B2 = 0.5 * (MaR - fabs(MaR));
+        G2 = B2 / MaR;
+    }
+    double MaI = B1 + B2;
+    double VI = MaI * CI;
+    bool up = VI >= 0.0;
+
+    double Dup = up ? DL : DR;
+    double Un = up ? UnL : UnR;
+    double vel[3];
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        if (d == dir)
+            vel[d] = Un;
+        else
+            vel[d] = muscl_upwind(up, LD(2 + d, -1), LD(2 + d, 0), LD(2 + d, 1), LD(2 + d, 2));
+    }
+    double Hup = muscl_upwind(up, LD(5, -1), LD(5, 0), LD(5, 1), LD(5, 2));
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+        double Y = muscl_upwind(up, LD(7 + s, -1), LD(7 + s, 0), LD(7 + s, 1), LD(7 + s, 2));
+        F[s] = VI * Dup * Y;
+    }
+    F[NS] = VI * Dup * vel[0];
+    F[NS + 1] = VI * Dup * vel[1];
+    F[NS + 2] = VI * Dup * vel[2];
+    F[NS + dir] = F[NS + dir] + G1 * PL + G2 * PR;
+    F[NS + 3] = VI * (Dup * Hup + 0.5 * Dup * ((vel[0] * vel[0]) + (vel[1] * vel[1]) + (vel[2] * vel[2])));
+#undef LD
+}
+
+// ---------------------------------------------------------------------------
+// Chemistry: rates (Reaction::Trapezoid / TrapezoidPrediction,
+// src/Reaction.cpp:603-631 and 875-911), reaction by reaction, accumulating
+// the per-species sums in reaction order (the reference's order for each
+// species).  Terms whose stoichiometric factor is zero add an exact zero in
+// the reference and are skipped.
+// ---------------------------------------------------------------------------
+
+// pow(x, y) for the exponents that occur, exact where the reference's
+// result is the correctly rounded one.
+__device__ __forceinline__ double pow_small(double x, double y)
+{
+    if (y == 0.0)
+        return 1.0;
+    if (y == 1.0)
+        return x;
+    if (y == 2.0)
+        return x * x;
+    if (y == -1.0)
+        return 1.0 / x;
+    return pow(x, y);
+}
+
+template <int NS>
+struct Rates {
+    double W[NS]; // Wi: mass production rate per species (g/cm^3/s)
+    double Q[NS]; // trapezoid production term, before the Mw factor
+    double P[NS]; // trapezoid loss term, before the division by Mc
+};
+
+// Mc in mol/cm^3.  If want_pq, Q and P sums are formed as well.
+template <int NS, bool want_pq>
+__device__ __forceinline__ void rates(const DevMech &m, double T, const double (&Mc)[NS], Rates<NS> &o)
+{
+    const int nr = m.nr;
+    const double R = m.R, Ru = m.Ru;
+    double Gi[NS];
+    {
+        TPow t = tpow(T);
+#pragma unroll
+        for (int s = 0; s < NS; s++) {
+            const double *a = coeffs(m, T, s);
+            double Hi = gethi(t, R, a, NS);
+            double Si = getsi(t, R, a, NS);
+            Gi[s] = Hi - Si * T;
+        }
+    }
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+        o.W[s] = 0.0;
+        if (want_pq) {
+            o.Q[s] = 0.0;
+            o.P[s] = 0.0;
+        }
+    }
+    for (int r = 0; r < nr; r++) {
+        double KF = m.af[r] * pow_small(T, m.bf[r]);
+        if (m.eaf[r] != 0.0)
+            KF = KF * exp(-m.eaf[r] / (Ru * T));
+        double delta_g = 0.0, RRF = 1.0, RRB = 1.0, RTB = 0.0;
+#pragma unroll
+        for (int s = 0; s < NS; s++) {
+            const double nb = m.nb[s * nr + r], nf = m.nf[s * nr + r], tb = m.tb[s * nr + r];
+            if (nb != nf)
+                delta_g += (nb - nf) * Gi[s];
+            if (nf != 0.0)
+                RRF *= pow_small(Mc[s], nf);
+            if (nb != 0.0)
+                RRB *= pow_small(Mc[s], nb);
+            if (tb != 0.0)
+                RTB += Mc[s] * tb;
+        }
+        double Kp = exp(-delta_g / (R * T)) * m.p0x[r];
+        double Kc = Kp * pow_small(R * T, m.mxr[r]);
+        double KB = KF / Kc;
+        if (RTB == 0.0)
+            RTB = 1.0;
+        double RR = KF * RRF - KB * RRB;
+#pragma unroll
+        for (int s = 0; s < NS; s++) {
+            const double nb = m.nb[s * nr + r], nf = m.nf[s * nr + r];
+            if (nb != nf)
+                o.W[s] += (nb - nf) * RTB * RR;
+            if (want_pq && (nb != 0.0 || nf != 0.0)) {
+                o.Q[s] += (nb * KF * RRF + nf * KB * RRB) * RTB;
+                o.P[s] += (nb * KB * RRB + nf * KF * RRF) * RTB;
+            }
+        }
+    }
+#pragma unroll
+    for (int s = 0; s < NS; s++)
+        o.W[s] = m.mw[s] * o.W[s];
+}
+
+// Sub-step counts above this are treated as invalid (health count) instead of
+// being run: a cell that needs more than a million sub-steps in one step has
+// left the regime the adaptive scheme is meant for, and running it would
+// stall the whole grid.
+constexpr double kMaxNchem = 1048576.0;
+
+// Nchem from the rates (src/Reaction.cpp:921-934); 0 flags an invalid value.
+template <int NS>
+__device__ __forceinline__ int predict_nchem(const double (&W)[NS], const double (&Di)[NS],
+                                             const double (&Y)[NS], double dtm)
+{
+    double dt = dtm;
+#pragma unroll
+    for (int s = 0; s < NS; s++)
+        if (Y[s] >= 1e-6)
+            if (W[s] != 0) {
+                double dt_temp = fabs(-Di[s] / W[s]);
+                if (dt_temp < dt)
+                    dt = dt_temp;
+            }
+    double n = ceil(dtm / dt);
+    if (!(n >= 1.0) || n > kMaxNchem)
+        return 0;
+    return (int)n;
+}
+
+} // namespace eu3d

@@ -1,0 +1,824 @@
+// eu3d_host.cpp -- host side: readers, mesh, initial conditions and the
+// time loop of the reactive-Euler step, driving the gfx950 kernels through
+// the C ABI (include/eu3d.h).  Built with -ffp-contract=off; the reference's
+// pow(x,2) / pow(x,-1) calls are written x*x and 1.0/x, which is what GCC
+// emits for them in the reference build.
+#include "eu3d_host.hpp"
+
+#include <cerrno>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <sstream>
+
+namespace eu3dh {
+
+static constexpr double kThermoSwitchT = 1000.0; // include/ThermoConstants.hpp:3
+
+void check(int rc, eu3d_ctx *ctx, const char *what)
+{
+    if (rc != EU3D_OK)
+        throw Error(rc, std::string(what) + ": " + (ctx ? eu3d_last_error(ctx) : "no context"));
+}
+
+// ===========================================================================
+// FileReader (src/FileReader.cpp:53-84)
+// ===========================================================================
+bool FileReader::readFile(const std::string &name)
+{
+    std::ifstream in(name);
+    if (!in)
+        return false;
+    std::string line, e1, e2;
+    while (std::getline(in, line)) {
+        std::istringstream ss(line);
+        ss >> e1 >> e2;
+        if (!e1.empty() && !e2.empty() && e1[0] != '#' && e2[0] != '#') {
+            const char *s = e2.c_str();
+            char *end;
+            errno = 0;
+            double v = std::strtod(s, &end);
+            if (end != s && errno != ERANGE) {
+                std::ostringstream o;
+                o << v; // default stream precision: 6 significant digits
+                parameters[e1] = o.str();
+            } else {
+                parameters[e1] = e2;
+            }
+        }
+    }
+    return true;
+}
+
+int FileReader::getIntParameter(const std::string &key) const
+{
+    auto it = parameters.find(key);
+    if (it == parameters.end())
+        throw Error(EU3D_ERR_ARG, "missing key " + key);
+    const char *s = it->second.c_str();
+    char *end;
+    errno = 0;
+    long v = std::strtol(s, &end, 10);
+    if (end == s || errno == ERANGE || v > 2147483647L || v < -2147483647L - 1)
+        return 0;
+    return (int)v;
+}
+
+double FileReader::getdoubleParameter(const std::string &key) const
+{
+    auto it = parameters.find(key);
+    if (it == parameters.end())
+        throw Error(EU3D_ERR_ARG, "missing key " + key);
+    const char *s = it->second.c_str();
+    char *end;
+    errno = 0;
+    double v = std::strtod(s, &end);
+    if (end == s || errno == ERANGE)
+        return 0;
+    return v;
+}
+
+std::string FileReader::getStringParameter(const std::string &key) const
+{
+    auto it = parameters.find(key);
+    if (it == parameters.end())
+        throw Error(EU3D_ERR_ARG, "missing key " + key);
+    return it->second;
+}
+
+// ===========================================================================
+// Mesh (src/Mesh.cpp:36-142), single block
+// ===========================================================================
+void Mesh::MeshProcess(const std::string &gridname)
+{
+    FileReader grid;
+    if (!grid.readFile(gridname))
+        throw Error(EU3D_ERR_ARG, "cannot open grid file " + gridname);
+    total_ni = grid.getIntParameter("total_ni");
+    xa = grid.getdoubleParameter("xa");
+    xb = grid.getdoubleParameter("xb");
+    total_nj = grid.getIntParameter("total_nj");
+    ya = grid.getdoubleParameter("ya");
+    yb = grid.getdoubleParameter("yb");
+    total_nk = grid.getIntParameter("total_nk");
+    za = grid.getdoubleParameter("za");
+    zb = grid.getdoubleParameter("zb");
+    bc = grid.getIntParameter("bc");
+    if (total_ni < 3 || total_nj < 3 || total_nk < 3 || bc < 2)
+        throw Error(EU3D_ERR_ARG, "grid needs >= 3 nodes per direction and bc >= 2");
+    dx = (xb - xa) / (total_ni - 1);
+    dy = (yb - ya) / (total_nj - 1);
+    dz = (zb - za) / (total_nk - 1);
+    ni = total_ni;
+    nj = total_nj;
+    nk = total_nk;
+    xnode.assign(ni + 2 * bc, 0.0);
+    ynode.assign(nj + 2 * bc, 0.0);
+    znode.assign(nk + 2 * bc, 0.0);
+    for (int i = bc; i < ni + bc; i++)
+        xnode[i] = xa + (0 * (ni - 1) + i - bc) * dx;
+    for (int j = bc; j < nj + bc; j++)
+        ynode[j] = ya + (0 * (nj - 1) + j - bc) * dy;
+    for (int k = bc; k < nk + bc; k++)
+        znode[k] = za + (0 * (nk - 1) + k - bc) * dz;
+    for (int i = 0; i < bc; i++) {
+        xnode[i] = 2 * xnode[bc] - xnode[2 * bc - i];
+        xnode[ni + bc + i] = 2 * xnode[ni + bc - 1] - xnode[ni + bc - 1 - i - 1];
+        ynode[i] = 2 * ynode[bc] - ynode[2 * bc - i];
+        ynode[nj + bc + i] = 2 * ynode[nj + bc - 1] - ynode[nj + bc - 1 - i - 1];
+        znode[i] = 2 * znode[bc] - znode[2 * bc - i];
+        znode[nk + bc + i] = 2 * znode[nk + bc - 1] - znode[nk + bc - 1 - i - 1];
+    }
+}
+
+// ===========================================================================
+// Reaction (src/Reaction.cpp:156-329, 1165-1220)
+// ===========================================================================
+static std::vector<std::string> read_lines(const std::string &path)
+{
+    std::ifstream in(path, std::ios::binary);
+    if (!in)
+        throw Error(EU3D_ERR_ARG, "cannot open " + path);
+    std::vector<std::string> lines;
+    std::string l;
+    while (std::getline(in, l))
+        lines.push_back(l);
+    return lines;
+}
+
+// whitespace tokens from line `from` on (seek_to_line + operator>>)
+class Tokens {
+  public:
+    Tokens(const std::vector<std::string> &L, int from) : L_(L), line_(from), pos_(0) {}
+    bool next(std::string &tok)
+    {
+        while (line_ < (int)L_.size()) {
+            const std::string &s = L_[line_];
+            while (pos_ < s.size() && std::isspace((unsigned char)s[pos_]))
+                pos_++;
+            if (pos_ < s.size()) {
+                size_t b = pos_;
+                while (pos_ < s.size() && !std::isspace((unsigned char)s[pos_]))
+                    pos_++;
+                tok = s.substr(b, pos_ - b);
+                return true;
+            }
+            line_++;
+            pos_ = 0;
+        }
+        return false;
+    }
+
+  private:
+    const std::vector<std::string> &L_;
+    int line_;
+    size_t pos_;
+};
+
+static double tok_double(const std::string &t)
+{
+    const char *s = t.c_str();
+    char *end;
+    double v = std::strtod(s, &end);
+    return end == s ? 0.0 : v;
+}
+static int tok_int(const std::string &t)
+{
+    const char *s = t.c_str();
+    char *end;
+    long v = std::strtol(s, &end, 10);
+    return end == s ? 0 : (int)v;
+}
+static std::vector<int> tab_ints(const std::string &row)
+{
+    std::vector<int> out;
+    size_t b = 0;
+    for (;;) {
+        size_t e = row.find('\t', b);
+        std::string item = row.substr(b, e == std::string::npos ? std::string::npos : e - b);
+        const char *s = item.c_str();
+        char *end;
+        long v = std::strtol(s, &end, 10);
+        if (end == s)
+            throw Error(EU3D_ERR_ARG, "bad integer item '" + item + "' in mechanism");
+        out.push_back((int)v);
+        if (e == std::string::npos)
+            break;
+        b = e + 1;
+    }
+    return out;
+}
+
+void Reaction::ReactionRead(const std::string &mech, const std::string &thermo)
+{
+    auto L = read_lines(mech);
+    int line = 2;
+    {
+        auto a = tab_ints(line < (int)L.size() ? L[line] : "");
+        a.resize(3, 0);
+        NS = a[0];
+        NR = a[1];
+        TB = a[2];
+    }
+    if (NS < 1 || NS > EU3D_MAX_NS || NR < 1 || NR > EU3D_MAX_NR)
+        throw Error(EU3D_ERR_ARG, "NS must be in [1,16] and NR in [1,24]");
+    Stoi_F.assign((size_t)NS * NR, 0.0);
+    Stoi_B.assign((size_t)NS * NR, 0.0);
+    React_TB.assign((size_t)NS * NR, 0.0);
+    Af.assign(NR, 0.0);
+    Bf.assign(NR, 0.0);
+    Eaf.assign(NR, 0.0);
+    Coeff0.assign(9 * (size_t)NS, 0.0);
+    Coeff1.assign(9 * (size_t)NS, 0.0);
+    Mw.assign(NS, 0.0);
+    Ri.assign(NS, 0.0);
+    Mr_temp.assign(NS, 0.0);
+    line = line + 6;
+    int cur = line;
+    for (int j = 0; j < NR; j++)
+        for (int side = 0; side < 2; side++) {
+            auto a = tab_ints(cur < (int)L.size() ? L[cur] : "");
+            cur++;
+            for (int i = 0; i < (int)a.size() && i < NS; i++)
+                (side == 0 ? Stoi_F : Stoi_B)[(size_t)i * NR + j] = a[i];
+        }
+    line = line + 2 * NR + 2;
+    {
+        Tokens ts(L, line);
+        std::string t;
+        for (int j = 0; j < NR; j++) {
+            double b[4] = {0, 0, 0, 0};
+            int i = 0;
+            while (ts.next(t)) {
+                b[i] = tok_double(t);
+                if (++i == 4)
+                    break;
+            }
+            Af[j] = b[1];
+            Bf[j] = b[2];
+            Eaf[j] = b[3];
+        }
+    }
+    line = line + NR + 3;
+    {
+        Tokens ts(L, line);
+        std::string t;
+        std::vector<int> a(100, 0);
+        for (int j = 0; j < TB; j++) {
+            int i = 0;
+            while (ts.next(t)) {
+                a[i] = tok_int(t);
+                if (++i == NS + 1)
+                    break;
+            }
+            if (a[0] < 1 || a[0] > NR)
+                throw Error(EU3D_ERR_ARG, "bad third-body row in mechanism");
+            for (int k = 0; k < NS; k++)
+                React_TB[(size_t)k * NR + (a[0] - 1)] = a[k + 1];
+        }
+    }
+    line = line + TB + 2;
+    {
+        Tokens ts(L, line);
+        std::string t;
+        int i = 0;
+        while (i < NS && ts.next(t))
+            Mr_temp[i++] = tok_double(t);
+    }
+    auto TL = read_lines(thermo);
+    line = 15;
+    for (int j = 0; j < NS; j++) {
+        Tokens ts(TL, line);
+        std::string t;
+        double aa[19] = {0};
+        int i = 0;
+        while (i < 19 && ts.next(t))
+            aa[i++] = tok_double(t);
+        for (int k = 0; k < 9; k++)
+            Coeff0[(size_t)k * NS + j] = aa[k];
+        for (int k = 0; k < 9; k++)
+            Coeff1[(size_t)k * NS + j] = aa[k + 9];
+        Mw[j] = aa[18];
+        line = line + 6;
+    }
+    for (int s = 0; s < NS; s++)
+        Ri[s] = R / Mw[s]; // src/Reaction.cpp:427-428
+}
+
+double Reaction::GetCpi(double T, double R, int SP) const
+{
+    const std::vector<double> &c = T < kThermoSwitchT ? Coeff0 : Coeff1;
+    auto a = [&](int k) { return c[(size_t)k * NS + SP]; };
+    return R * (a(0) * std::pow(T, -2) + a(1) * (1.0 / T) + a(2) + a(3) * T + a(4) * (T * T) +
+                a(5) * std::pow(T, 3) + a(6) * std::pow(T, 4));
+}
+
+double Reaction::GetHi(double T, double R, int SP) const
+{
+    const std::vector<double> &c = T < kThermoSwitchT ? Coeff0 : Coeff1;
+    auto a = [&](int k) { return c[(size_t)k * NS + SP]; };
+    return R * (-a(0) * (1.0 / T) + a(1) * std::log(T) + a(2) * T + a(3) * (T * T) / 2 + a(4) * std::pow(T, 3) / 3 +
+                a(5) * std::pow(T, 4) / 4 + a(6) * std::pow(T, 5) / 5 + a(7));
+}
+
+double Reaction::GetSi(double T, double R, int SP) const
+{
+    const std::vector<double> &c = T < kThermoSwitchT ? Coeff0 : Coeff1;
+    auto a = [&](int k) { return c[(size_t)k * NS + SP]; };
+    return R * (-a(0) * std::pow(T, -2) / 2 - a(1) * (1.0 / T) + a(2) * std::log(T) + a(3) * T +
+                a(4) * (T * T) / 2 + a(5) * std::pow(T, 3) / 3 + a(6) * std::pow(T, 4) / 4 + a(8));
+}
+
+eu3d_mechanism Reaction::Mechanism() const
+{
+    eu3d_mechanism m;
+    m.ns = NS;
+    m.nr = NR;
+    m.R = R;
+    m.Ru = Ru;
+    m.P0 = P0;
+    m.mw = Mw.data();
+    m.ri = Ri.data();
+    m.coeff0 = Coeff0.data();
+    m.coeff1 = Coeff1.data();
+    m.af = Af.data();
+    m.bf = Bf.data();
+    m.eaf = Eaf.data();
+    m.stoi_f = Stoi_F.data();
+    m.stoi_b = Stoi_B.data();
+    m.react_tb = React_TB.data();
+    return m;
+}
+
+// ===========================================================================
+// Flowfield
+// ===========================================================================
+void Flowfield::InputRead(const std::string &initialization, const Mesh &m)
+{
+    mesh = &m;
+    ni = m.ni;
+    nj = m.nj;
+    nk = m.nk;
+    bc = m.bc;
+    NI = ni + 2 * bc;
+    NJ = nj + 2 * bc;
+    NK = nk + 2 * bc;
+    N = (size_t)NI * NJ * NK;
+    if (!input.readFile(initialization))
+        throw Error(EU3D_ERR_ARG, "cannot open input file " + initialization);
+    NS = input.getIntParameter("NS");
+    NR = input.getIntParameter("NR");
+    P_pre = input.getdoubleParameter("P_pre");
+    T_pre = input.getdoubleParameter("T_pre");
+    M_wave = input.getdoubleParameter("M_wave");
+    case_name = input.has("case") ? input.getStringParameter("case") : "RSBI";
+}
+
+void Flowfield::Construction()
+{
+    for (auto *v : {&U, &V, &W, &P, &D, &T, &C, &Ma, &Wav, &Rgas, &Cp, &H, &E, &Gamma})
+        v->assign(N, 0.0);
+    for (auto *v : {&Mc, &Mi, &Yi, &Di, &CMS})
+        v->assign(N * NS, 0.0);
+    CS.assign(N * (NS + 4), 0.0);
+    Nchem.assign(N, 1); // src/Reaction.cpp:102
+}
+
+// RSBI (src/Reaction.cpp:412-471, src/Flowfield.cpp:204-225)
+void Flowfield::ic_rsbi(const Reaction &r)
+{
+    const auto &x = mesh->xnode, &y = mesh->ynode;
+    double cx = 0.038, cy = 0.0, Ir = 0.022, Or = 1.2 * Ir, alpha = 5;
+    for (int i = 0; i < NI; i++)
+        for (int j = 0; j < NJ; j++)
+            for (int k = 0; k < NK; k++) {
+                size_t id = idx(i, j, k);
+                double *Yc = &Yi[id * NS], *Mic = &Mi[id * NS];
+                double Mr[EU3D_MAX_NS] = {0};
+                double Mr_SUM = 0.0, Total_Mw = 0.0;
+                double ddx = x[i] - cx, ddy = y[j] - cy;
+                if (ddx * ddx + ddy * ddy > Or * Or || ddx * ddx + ddy * ddy < Ir * Ir) {
+                    if (ddx * ddx + ddy * ddy > Or * Or) {
+                        Mr[4] = 1.0;
+                        Mr[8] = 3.76;
+                    } else {
+                        Mr[5] = 1.0;
+                    }
+                    for (int s = 0; s < NS; s++) {
+                        Mr_SUM += Mr[s];
+                        Total_Mw += Mr[s] * r.Mw[s];
+                    }
+                    for (int s = 0; s < NS; s++) {
+                        Mic[s] = Mr[s] / Mr_SUM;
+                        Yc[s] = Mr[s] * r.Mw[s] / Total_Mw;
+                    }
+                } else {
+                    double rr = std::sqrt(ddx * ddx + ddy * ddy);
+                    double t = (rr - Ir) / (Or - Ir);
+                    Yc[5] = std::exp(-alpha * (t * t));
+                    Yc[4] = (1 - Yc[5]) * r.Mw[4] / (r.Mw[4] + 3.76 * r.Mw[8]);
+                    Yc[8] = (1 - Yc[5]) * 3.76 * r.Mw[8] / (r.Mw[4] + 3.76 * r.Mw[8]);
+                }
+            }
+    double M = M_wave;
+    double U_post = M * (1 - (2 + 0.40 * (M * M)) / (2.40 * (M * M))) * std::sqrt(1.40 * 287 * T_pre);
+    double P_post = P_pre * (1 + 2 * 1.4 / 2.4 * ((M * M) - 1.0));
+    double T_post = T_pre * (2 + 0.40 * (M * M)) / (2.40 * (M * M)) * P_post / P_pre;
+    for (int i = 0; i < NI; i++)
+        for (int j = 0; j < NJ; j++)
+            for (int k = 0; k < NK; k++) {
+                size_t id = idx(i, j, k);
+                V[id] = 0.0;
+                W[id] = 0.0;
+                if (x[i] >= 0.01) {
+                    P[id] = P_pre;
+                    T[id] = T_pre;
+                    U[id] = 0.0;
+                } else {
+                    P[id] = P_post;
+                    T[id] = T_post;
+                    U[id] = U_post;
+                }
+            }
+}
+
+// splitmix64 counter-based generator for the seeded interface perturbation
+static uint64_t splitmix64(uint64_t x)
+{
+    uint64_t z = x + 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+static double u01(uint64_t seed, uint64_t ctr)
+{
+    return (double)(splitmix64(seed * 0x100000001B3ull + ctr) >> 11) * (1.0 / 9007199254740992.0);
+}
+
+// Spherical shock-bubble ("SBI"): planar moving normal shock in N2 (gamma 1.4
+// jump relations with the N2 gas constant) hitting a stoichiometric H2-O2
+// bubble, tanh-smoothed interface, seeded radius perturbation.
+void Flowfield::ic_sbi(const Reaction &r)
+{
+    const Mesh &m = *mesh;
+    double xr = input.getdoubleDefault("bubble_xc_frac", 0.35);
+    double rr = input.getdoubleDefault("bubble_r_frac", 0.2);
+    double gap = input.getdoubleDefault("shock_gap_cells", 10.0);
+    double wc = input.getdoubleDefault("tanh_cells", 2.0);
+    double amp = input.getdoubleDefault("perturb_amp", 0.01);
+    uint64_t seed = (uint64_t)input.getdoubleDefault("seed", 20240601.0);
+    double dx = (m.xb - m.xa) / (m.total_ni - 1);
+    double R0 = rr * (m.yb - m.ya);
+    double xc = m.xa + xr * (m.xb - m.xa);
+    double yc = 0.5 * (m.ya + m.yb), zc = 0.5 * (m.za + m.zb);
+    double xs = xc - R0 - gap * dx;
+    double width = wc * dx;
+    double nq[8][3], lq[8], phq[8];
+    for (int q = 0; q < 8; q++) {
+        double zz = 2.0 * u01(seed, 4 * q + 0) - 1.0;
+        double az = 6.283185307179586 * u01(seed, 4 * q + 1);
+        double rho = std::sqrt(1.0 - zz * zz);
+        nq[q][0] = rho * std::cos(az);
+        nq[q][1] = rho * std::sin(az);
+        nq[q][2] = zz;
+        lq[q] = 2.0 + (double)(splitmix64(seed * 31u + 7u + (uint64_t)q) % 3u);
+        phq[q] = 6.283185307179586 * u01(seed, 4 * q + 3);
+    }
+    double mwH2 = r.Mw[5], mwO2 = r.Mw[4];
+    double YH2b = 2.0 * mwH2 / (2.0 * mwH2 + mwO2);
+    double YO2b = mwO2 / (2.0 * mwH2 + mwO2);
+    double M = M_wave;
+    double Rn2 = r.R * 1000 / r.Mw[8];
+    double U_post = M * (1 - (2 + 0.40 * (M * M)) / (2.40 * (M * M))) * std::sqrt(1.40 * Rn2 * T_pre);
+    double P_post = P_pre * (1 + 2 * 1.4 / 2.4 * ((M * M) - 1.0));
+    double T_post = T_pre * (2 + 0.40 * (M * M)) / (2.40 * (M * M)) * P_post / P_pre;
+    for (int i = 0; i < NI; i++)
+        for (int j = 0; j < NJ; j++)
+            for (int k = 0; k < NK; k++) {
+                size_t id = idx(i, j, k);
+                double ex = m.xnode[i] - xc, ey = m.ynode[j] - yc, ez = m.znode[k] - zc;
+                double rad = std::sqrt(ex * ex + ey * ey + ez * ez);
+                double p = 0.0;
+                if (rad > 0.0) {
+                    for (int q = 0; q < 8; q++) {
+                        double c = (ex * nq[q][0] + ey * nq[q][1] + ez * nq[q][2]) / rad;
+                        p += std::cos(3.141592653589793 * lq[q] * c + phq[q]);
+                    }
+                    p = p / 8.0;
+                }
+                double Rloc = R0 * (1.0 + amp * p);
+                double phi = 0.5 * (1.0 - std::tanh((rad - Rloc) / width));
+                double *Yc = &Yi[id * NS];
+                for (int s = 0; s < NS; s++)
+                    Yc[s] = 0.0;
+                Yc[5] = phi * YH2b;
+                Yc[4] = phi * YO2b;
+                Yc[8] = 1.0 - phi;
+                V[id] = 0.0;
+                W[id] = 0.0;
+                if (m.xnode[i] < xs) {
+                    P[id] = P_post;
+                    T[id] = T_post;
+                    U[id] = U_post;
+                } else {
+                    P[id] = P_pre;
+                    T[id] = T_pre;
+                    U[id] = 0.0;
+                }
+            }
+}
+
+// 1-D ZND initiation ("ZND"): stoichiometric H2-air at rest, driver of burnt
+// products (2 H2O + 3.76 N2) over the left fraction of the domain.
+void Flowfield::ic_znd(const Reaction &r)
+{
+    const Mesh &m = *mesh;
+    double frac = input.getdoubleDefault("driver_frac", 0.05);
+    double Pd = input.getdoubleDefault("driver_P", 30.0 * 101325.0);
+    double Td = input.getdoubleDefault("driver_T", 3000.0);
+    double xd = m.xa + frac * (m.xb - m.xa);
+    double mw_u = 2.0 * r.Mw[5] + r.Mw[4] + 3.76 * r.Mw[8];
+    double mw_b = 2.0 * r.Mw[2] + 3.76 * r.Mw[8];
+    for (int i = 0; i < NI; i++)
+        for (int j = 0; j < NJ; j++)
+            for (int k = 0; k < NK; k++) {
+                size_t id = idx(i, j, k);
+                double *Yc = &Yi[id * NS];
+                for (int s = 0; s < NS; s++)
+                    Yc[s] = 0.0;
+                U[id] = 0.0;
+                V[id] = 0.0;
+                W[id] = 0.0;
+                if (m.xnode[i] < xd) {
+                    Yc[2] = 2.0 * r.Mw[2] / mw_b;
+                    Yc[8] = 3.76 * r.Mw[8] / mw_b;
+                    P[id] = Pd;
+                    T[id] = Td;
+                } else {
+                    Yc[5] = 2.0 * r.Mw[5] / mw_u;
+                    Yc[4] = r.Mw[4] / mw_u;
+                    Yc[8] = 3.76 * r.Mw[8] / mw_u;
+                    P[id] = P_pre;
+                    T[id] = T_pre;
+                }
+            }
+}
+
+// Flowfield::FieldInitial (src/Flowfield.cpp:193-264) for the selected case
+void Flowfield::FieldInitial(const Reaction &r)
+{
+    if (NS != r.NS || NR != r.NR)
+        throw Error(EU3D_ERR_ARG, "input NS/NR do not match the mechanism");
+    if (case_name == "RSBI")
+        ic_rsbi(r);
+    else if (case_name == "SBI")
+        ic_sbi(r);
+    else if (case_name == "ZND")
+        ic_znd(r);
+    else
+        throw Error(EU3D_ERR_ARG, "unknown case " + case_name);
+    const double R = r.R;
+    std::vector<double> Cpi(NS), Hi(NS), Ei(NS), Y(NS);
+    auto sum2 = [&](const std::vector<double> &a, const std::vector<double> &b) {
+        double s = 0;
+        for (int q = 0; q < NS; q++)
+            s += (a[q] * b[q]);
+        return s;
+    };
+    auto sum3 = [&](const std::vector<double> &a, const std::vector<double> &b) {
+        double s = 0;
+        for (int q = 0; q < NS; q++)
+            s += (a[q] / b[q]);
+        return s;
+    };
+    for (int i = bc; i < ni + bc; i++)
+        for (int j = bc; j < nj + bc; j++)
+            for (int k = bc; k < nk + bc; k++) {
+                size_t id = idx(i, j, k);
+                for (int s = 0; s < NS; s++)
+                    Y[s] = Yi[id * NS + s];
+                Wav[id] = 1.0 / sum3(Y, r.Mw);
+                Rgas[id] = R * 1000 / Wav[id];
+                D[id] = P[id] / (Rgas[id] * T[id]);
+                for (int s = 0; s < NS; s++) {
+                    Di[id * NS + s] = Yi[id * NS + s] * D[id];
+                    Mc[id * NS + s] = Di[id * NS + s] / r.Mw[s] * 1000;
+                    Cpi[s] = r.GetCpi(T[id], r.Ri[s], s);
+                    Hi[s] = r.GetHi(T[id], r.Ri[s], s);
+                    Ei[s] = Hi[s] - r.Ri[s] * T[id];
+                }
+                Cp[id] = sum2(Cpi, Y) * 1000;
+                H[id] = sum2(Y, Hi) * 1000;
+                E[id] = sum2(Y, Ei) * 1000;
+                Gamma[id] = Cp[id] / (Cp[id] - R * sum3(Y, r.Mw) * 1000);
+                C[id] = std::sqrt(Gamma[id] * P[id] / D[id]);
+                double *cs = &CS[id * (NS + 4)];
+                double u = U[id], v = V[id], w = W[id], d = D[id];
+                for (int s = 0; s < NS; s++)
+                    cs[s] = d * Yi[id * NS + s];
+                cs[NS] = d * u;
+                cs[NS + 1] = d * v;
+                cs[NS + 2] = d * w;
+                cs[NS + 3] = d * E[id] + 0.5 * ((u * u) + (v * v) + (w * w)) * d;
+            }
+}
+
+eu3d_state Flowfield::HostState()
+{
+    eu3d_state s;
+    s.U = U.data();
+    s.V = V.data();
+    s.W = W.data();
+    s.P = P.data();
+    s.D = D.data();
+    s.T = T.data();
+    s.C = C.data();
+    s.Ma = Ma.data();
+    s.Wav = Wav.data();
+    s.Rgas = Rgas.data();
+    s.Cp = Cp.data();
+    s.H = H.data();
+    s.E = E.data();
+    s.Gamma = Gamma.data();
+    s.Mc = Mc.data();
+    s.Yi = Yi.data();
+    s.Di = Di.data();
+    s.CMS = CMS.data();
+    s.CS = CS.data();
+    s.Nchem = Nchem.data();
+    return s;
+}
+
+void Flowfield::Upload()
+{
+    eu3d_state s = HostState();
+    check(eu3d_upload_state(ctx, &s), ctx, "eu3d_upload_state");
+}
+
+void Flowfield::Download()
+{
+    eu3d_state s = HostState();
+    check(eu3d_download_state(ctx, &s), ctx, "eu3d_download_state");
+}
+
+void Flowfield::FieldBoundary_3dRSBI() { check(eu3d_fill_ghosts(ctx), ctx, "eu3d_fill_ghosts"); }
+
+void Flowfield::CFLcondition(double cfl, double Final_Time)
+{
+    check(eu3d_cfl_dt(ctx, cfl, time, Final_Time, &dt), ctx, "eu3d_cfl_dt");
+    time += dt;
+}
+
+void Flowfield::Advection(int time_adv)
+{
+    if (time_adv == EU3D_SCHEME_EE)
+        TimeAdv::EE(ctx, dt);
+    else
+        TimeAdv::TVD_RK3(ctx, dt);
+}
+
+void Flowfield::Update_after_Adv() { check(eu3d_recover_state(ctx), ctx, "eu3d_recover_state"); }
+
+// ===========================================================================
+// TimeAdv
+// ===========================================================================
+void TimeAdv::EE(eu3d_ctx *ctx, double dt) { check(eu3d_flux_update(ctx, dt), ctx, "eu3d_flux_update"); }
+
+void TimeAdv::TVD_RK3(eu3d_ctx *ctx, double dt) { check(eu3d_advance_rk3(ctx, dt), ctx, "eu3d_advance_rk3"); }
+
+// ===========================================================================
+// Euler (src/Euler.cpp:59-225, 306-421)
+// ===========================================================================
+std::string Euler::path(const std::string &p) const
+{
+    if (p.empty() || p[0] == '/' || base.empty())
+        return p;
+    return base + "/" + p;
+}
+
+Euler::Euler(const std::string &ctrlfile, const std::string &base_dir, bool device) : base(base_dir)
+{
+    FileReader Ctrl;
+    if (!Ctrl.readFile(ctrlfile))
+        throw Error(EU3D_ERR_ARG, "cannot open ctrl file " + ctrlfile);
+    Final_Time = Ctrl.getdoubleParameter("Final-Time");
+    cfl = Ctrl.getdoubleParameter("cfl");
+    if (cfl == 0)
+        dt = Ctrl.getdoubleParameter("dt");
+    count = Ctrl.getIntParameter("count");
+    if (Ctrl.getStringParameter("reaction") != "Trapezoid-Adaptive")
+        throw Error(EU3D_ERR_ARG, "reaction scheme must be Trapezoid-Adaptive");
+    if (Ctrl.getStringParameter("difference") != "MUSCL_2")
+        throw Error(EU3D_ERR_ARG, "difference scheme must be MUSCL_2");
+    std::string ta = Ctrl.getStringParameter("time-adv");
+    if (ta == "EE")
+        TimeAdv_Sch = EU3D_SCHEME_EE;
+    else if (ta == "TVD_RK3")
+        TimeAdv_Sch = EU3D_SCHEME_TVD_RK3;
+    else
+        throw Error(EU3D_ERR_ARG, "time-adv must be EE or TVD_RK3");
+    Mymesh.MeshProcess(path(Ctrl.getStringParameter("gridname")));
+    TwoDim.InputRead(path(Ctrl.getStringParameter("initialization")), Mymesh);
+    React.ReactionRead(path(Ctrl.getStringParameter("reaction-model")), path(Ctrl.getStringParameter("Thermofile")));
+    TwoDim.Construction();
+    TwoDim.FieldInitial(React);
+    if (device)
+        Attach();
+}
+
+void Euler::Attach()
+{
+    if (TwoDim.ctx)
+        return;
+    eu3d_mesh mesh{Mymesh.ni, Mymesh.nj, Mymesh.nk, Mymesh.bc, Mymesh.xnode.data(), Mymesh.ynode.data(),
+                   Mymesh.znode.data()};
+    eu3d_mechanism mech = React.Mechanism();
+    int rc = eu3d_create(&mesh, &mech, &TwoDim.ctx);
+    if (rc != EU3D_OK) {
+        std::string msg = TwoDim.ctx ? eu3d_last_error(TwoDim.ctx) : "eu3d_create failed";
+        eu3d_destroy(TwoDim.ctx);
+        TwoDim.ctx = nullptr;
+        throw Error(rc, msg);
+    }
+    TwoDim.Upload();
+}
+
+Euler::~Euler()
+{
+    if (TwoDim.ctx)
+        eu3d_destroy(TwoDim.ctx);
+}
+
+// Euler::Computing (src/Euler.cpp:306-421): the loop body of one iteration is
+// boundary -> advection -> post-advection state -> chemistry.
+void Euler::Computing(int max_steps)
+{
+    const double time_tol = 1e-12 * std::fabs(Final_Time);
+    int n = 0;
+    while (Final_Time - time > time_tol && n < max_steps) {
+        if (cfl == 0)
+            TwoDim.dt = dt;
+        else
+            TwoDim.CFLcondition(cfl, Final_Time);
+        if (TwoDim.dt > Final_Time - time)
+            TwoDim.dt = Final_Time - time;
+        check(eu3d_step(TwoDim.ctx, TimeAdv_Sch, TwoDim.dt, chem_flags), TwoDim.ctx, "eu3d_step");
+        time += TwoDim.dt;
+        iteration += 1;
+        n++;
+    }
+}
+
+void Euler::SyncOutput() { TwoDim.Download(); }
+
+// EU3D_STATE_V1_LE dump (src/Euler.cpp:1539-1598), single process
+void Euler::WriteStateDump(const std::string &p, int num)
+{
+    SyncOutput();
+    std::FILE *f = std::fopen(p.c_str(), "wb");
+    if (!f)
+        throw Error(EU3D_ERR_ARG, "cannot open dump " + p);
+    const int NS = React.NS, bc = Mymesh.bc;
+    double dtm = cfl == 0 ? dt : TwoDim.dt;
+    std::fprintf(f, "EU3D_STATE_V1_LE\n");
+    std::fprintf(f, "%d %d %d %d %d %d %d 0 0 0 1 1 1 0 1 %d %d %.17g %.17g\n", NS, Mymesh.total_ni,
+                 Mymesh.total_nj, Mymesh.total_nk, Mymesh.ni, Mymesh.nj, Mymesh.nk, iteration, num, Final_Time, dtm);
+    std::fprintf(f, "X Y Z D T U V W E Wav Rgas Cp P H Gamma C Ma");
+    for (const char *name : {"Di", "Yi", "Mc"})
+        for (int s = 0; s < NS; s++)
+            std::fprintf(f, " %s%d", name, s);
+    for (int s = 0; s < NS + 4; s++)
+        std::fprintf(f, " CS%d", s);
+    for (int s = 0; s < NS; s++)
+        std::fprintf(f, " CMS%d", s);
+    std::fprintf(f, " Nchem\n");
+    Flowfield &F = TwoDim;
+    std::vector<double> row;
+    for (int i = bc; i < bc + Mymesh.ni; i++)
+        for (int j = bc; j < bc + Mymesh.nj; j++)
+            for (int k = bc; k < bc + Mymesh.nk; k++) {
+                size_t id = F.idx(i, j, k);
+                row = {Mymesh.xnode[i], Mymesh.ynode[j], Mymesh.znode[k], F.D[id], F.T[id], F.U[id],
+                       F.V[id],         F.W[id],         F.E[id],         F.Wav[id], F.Rgas[id], F.Cp[id],
+                       F.P[id],         F.H[id],         F.Gamma[id],     F.C[id],   F.Ma[id]};
+                for (int s = 0; s < NS; s++)
+                    row.push_back(F.Di[id * NS + s]);
+                for (int s = 0; s < NS; s++)
+                    row.push_back(F.Yi[id * NS + s]);
+                for (int s = 0; s < NS; s++)
+                    row.push_back(F.Mc[id * NS + s]);
+                for (int s = 0; s < NS + 4; s++)
+                    row.push_back(F.CS[id * (NS + 4) + s]);
+                for (int s = 0; s < NS; s++)
+                    row.push_back(F.CMS[id * NS + s]);
+                row.push_back((double)F.Nchem[id]);
+                std::fwrite(row.data(), sizeof(double), row.size(), f);
+            }
+    if (std::fclose(f) != 0)
+        throw Error(EU3D_ERR_ARG, "dump write failed " + p);
+}
+
+} // namespace eu3dh

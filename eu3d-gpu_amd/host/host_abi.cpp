@@ -1,0 +1,157 @@
+// host_abi.cpp -- extern "C" wrapper of the host driver (include/eu3d_host.h).
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+#include "../../include/eu3d_host.h"
+#include "eu3d_host.hpp"
+
+using eu3dh::Error;
+
+struct eu3d_solver {
+    eu3dh::Euler *e = nullptr;
+    std::string err;
+};
+
+#define GUARD(body)                                                                               \
+    try {                                                                                         \
+        body;                                                                                     \
+    } catch (const Error &x) {                                                                    \
+        s->err = x.what();                                                                        \
+        return x.code;                                                                            \
+    } catch (const std::exception &x) {                                                           \
+        s->err = x.what();                                                                        \
+        return EU3D_ERR_ARG;                                                                      \
+    }                                                                                             \
+    return EU3D_OK;
+
+extern "C" {
+
+static int open_impl(const char *ctrl, const char *base_dir, eu3d_solver **out, char *err, int errlen, bool device)
+{
+    if (!ctrl || !out)
+        return EU3D_ERR_ARG;
+    *out = nullptr;
+    try {
+        auto *e = new eu3dh::Euler(ctrl, base_dir ? base_dir : "", device);
+        *out = new eu3d_solver();
+        (*out)->e = e;
+    } catch (const Error &x) {
+        if (err && errlen > 0)
+            std::snprintf(err, errlen, "%s", x.what());
+        return x.code;
+    } catch (const std::exception &x) {
+        if (err && errlen > 0)
+            std::snprintf(err, errlen, "%s", x.what());
+        return EU3D_ERR_ARG;
+    }
+    return EU3D_OK;
+}
+
+int eu3d_solver_open(const char *ctrl, const char *base_dir, eu3d_solver **out, char *err, int errlen)
+{
+    return open_impl(ctrl, base_dir, out, err, errlen, true);
+}
+
+int eu3d_solver_open_host(const char *ctrl, const char *base_dir, eu3d_solver **out, char *err, int errlen)
+{
+    return open_impl(ctrl, base_dir, out, err, errlen, false);
+}
+
+int eu3d_solver_attach(eu3d_solver *s)
+{
+    if (!s)
+        return EU3D_ERR_ARG;
+    GUARD(s->e->Attach());
+}
+
+int eu3d_solver_close(eu3d_solver *s)
+{
+    if (!s)
+        return EU3D_OK;
+    delete s->e;
+    delete s;
+    return EU3D_OK;
+}
+
+eu3d_ctx *eu3d_solver_ctx(eu3d_solver *s) { return s && s->e ? s->e->TwoDim.ctx : nullptr; }
+
+int eu3d_solver_dims(const eu3d_solver *s, int *d)
+{
+    if (!s || !d)
+        return EU3D_ERR_ARG;
+    const auto &F = s->e->TwoDim;
+    int v[10] = {F.ni, F.nj, F.nk, F.bc, F.NI, F.NJ, F.NK, s->e->React.NS, s->e->React.NR, s->e->TimeAdv_Sch};
+    std::memcpy(d, v, sizeof v);
+    return EU3D_OK;
+}
+
+int eu3d_solver_info(const eu3d_solver *s, double *o)
+{
+    if (!s || !o)
+        return EU3D_ERR_ARG;
+    o[0] = s->e->Final_Time;
+    o[1] = s->e->cfl;
+    o[2] = s->e->dt;
+    o[3] = s->e->time;
+    o[4] = s->e->TwoDim.dt;
+    o[5] = s->e->iteration;
+    return EU3D_OK;
+}
+
+int eu3d_solver_run(eu3d_solver *s, int max_steps, int chem_flags)
+{
+    if (!s)
+        return EU3D_ERR_ARG;
+    GUARD(s->e->SetChemFlags(chem_flags); s->e->Computing(max_steps));
+}
+
+int eu3d_solver_sync(eu3d_solver *s)
+{
+    if (!s)
+        return EU3D_ERR_ARG;
+    GUARD(s->e->SyncOutput());
+}
+
+int eu3d_solver_upload(eu3d_solver *s)
+{
+    if (!s)
+        return EU3D_ERR_ARG;
+    GUARD(s->e->TwoDim.Upload());
+}
+
+double *eu3d_solver_field(eu3d_solver *s, const char *n)
+{
+    if (!s || !n)
+        return nullptr;
+    auto &F = s->e->TwoDim;
+    auto &R = s->e->React;
+    auto &M = s->e->Mymesh;
+    struct {
+        const char *n;
+        std::vector<double> *v;
+    } t[] = {{"U", &F.U},         {"V", &F.V},         {"W", &F.W},       {"P", &F.P},       {"D", &F.D},
+             {"T", &F.T},         {"C", &F.C},         {"Ma", &F.Ma},     {"Wav", &F.Wav},   {"Rgas", &F.Rgas},
+             {"Cp", &F.Cp},       {"H", &F.H},         {"E", &F.E},       {"Gamma", &F.Gamma}, {"Mc", &F.Mc},
+             {"Mi", &F.Mi},       {"Yi", &F.Yi},       {"Di", &F.Di},     {"CMS", &F.CMS},   {"CS", &F.CS},
+             {"xnode", &M.xnode}, {"ynode", &M.ynode}, {"znode", &M.znode}, {"coeff0", &R.Coeff0},
+             {"coeff1", &R.Coeff1}, {"mw", &R.Mw},     {"ri", &R.Ri},     {"stoi_f", &R.Stoi_F},
+             {"stoi_b", &R.Stoi_B}, {"react_tb", &R.React_TB}, {"af", &R.Af}, {"bf", &R.Bf}, {"eaf", &R.Eaf}};
+    for (auto &e : t)
+        if (std::strcmp(e.n, n) == 0)
+            return e.v->data();
+    return nullptr;
+}
+
+int *eu3d_solver_nchem(eu3d_solver *s) { return s ? s->e->TwoDim.Nchem.data() : nullptr; }
+
+int eu3d_solver_dump(eu3d_solver *s, const char *path, int num)
+{
+    if (!s || !path)
+        return EU3D_ERR_ARG;
+    GUARD(s->e->WriteStateDump(path, num));
+}
+
+const char *eu3d_solver_error(const eu3d_solver *s) { return s ? s->err.c_str() : "null solver"; }
+
+} // extern "C"

@@ -1,0 +1,167 @@
+/*
+ * eu3d.h -- C ABI of the MI355X (gfx950) reactive-Euler step.
+ *
+ * Plain pointers and sizes only.  One context owns all device memory of one
+ * solve and one HIP stream; calls on one context are not thread-safe.  Every
+ * call returns an int status (EU3D_OK or a negative EU3D_ERR_*); nothing in
+ * the library exits the process.  eu3d_last_error() describes the last failure.
+ *
+ * The entry points replace the reference's CUDA boundary (the include/ .cuh headers) and
+ * the CPU loops behind its Flowfield / TimeAdv / Reaction classes; each
+ * declaration names what it replaces.
+ *
+ * Host arrays use the reference layout (include/Array.hpp:33-43): row-major
+ * over the padded grid NI x NJ x NK (NI = ni + 2*bc), last index fastest;
+ * species fields are (i,j,k,s) with s fastest; the conserved vector CS is
+ * (i,j,k,n) with n in [0, ns+4): ns partial densities, rho*u, rho*v, rho*w,
+ * total energy.
+ */
+#ifndef EU3D_H
+#define EU3D_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define EU3D_OK 0
+#define EU3D_ERR_ARG (-1)    /* invalid argument, unsupported size, dt <= 0 (cudaErrorInvalidValue) */
+#define EU3D_ERR_HIP (-2)    /* a HIP runtime call failed */
+#define EU3D_ERR_STATE (-3)  /* call out of order (e.g. chemistry before any state upload) */
+#define EU3D_ERR_HEALTH (-4) /* invalid Nchem or non-finite state found (reference: "wrong chem" abort) */
+
+#define EU3D_MAX_NS 16 /* include/ReactionCuda.cuh:4 */
+#define EU3D_MAX_NR 24 /* include/ReactionCuda.cuh:5 */
+
+#define EU3D_SCHEME_EE 0      /* Global::EE       (include/Global.hpp) */
+#define EU3D_SCHEME_TVD_RK3 1 /* Global::TVD_RK3  (a stub in the reference, built here) */
+
+#define EU3D_CHEM_COMPACT 0x1   /* stiff cells (Nchem > 1) are binned into a dense work queue */
+#define EU3D_CHEM_SKIP 0x2      /* advection only (no chemistry) */
+#define EU3D_CHEM_KEEP_CMS 0x4  /* store the last sub-step mass production rates (dump field CMS) */
+
+typedef struct eu3d_ctx eu3d_ctx;
+
+/* Mesh nodes incl. mirrored ghosts (src/Mesh.cpp:96-142). */
+typedef struct {
+    int ni, nj, nk, bc; /* interior nodes per direction (each >= 3), ghost layers (>= 2) */
+    const double *xnode, *ynode, *znode; /* ni+2bc, nj+2bc, nk+2bc */
+} eu3d_mesh;
+
+/* Mechanism and thermo tables as Reaction::ReactionRead builds them
+ * (src/Reaction.cpp:156-329, include/Reaction.hpp:44-78). */
+typedef struct {
+    int ns, nr;
+    double R, Ru, P0;                         /* 8.31434 J/mol/K, 1.987 cal/mol/K, 101325 Pa */
+    const double *mw, *ri;                    /* [ns] g/mol, kJ/kg/K (R/Mw) */
+    const double *coeff0, *coeff1;            /* [9*ns], index k*ns+s; T < 1000 K and T >= 1000 K */
+    const double *af, *bf, *eaf;              /* [nr] Arrhenius A, b, E (cal/mol) */
+    const double *stoi_f, *stoi_b, *react_tb; /* [ns*nr], index s*nr+r */
+} eu3d_mechanism;
+
+/* Flow state in the reference layout (include/Flowfield.hpp:68-110).  On
+ * upload, U..Gamma, Yi, T, E are read for every point (ghosts included), CS
+ * for every point; NULL fields are not read (Ma, Wav, Rgas, Cp, C, Mc, Di,
+ * CMS, Nchem may be NULL).  On download any pointer may be NULL. */
+typedef struct {
+    double *U, *V, *W, *P, *D, *T, *C, *Ma, *Wav, *Rgas, *Cp, *H, *E, *Gamma; /* [N] */
+    double *Mc, *Yi, *Di, *CMS;                                                /* [N*ns] */
+    double *CS;                                                                /* [N*(ns+4)] */
+    int *Nchem;                                                                /* [N] */
+} eu3d_state;
+
+/* Health of the last chemistry call (src/ReactionCuda.cu:2265-2273). */
+typedef struct {
+    long long invalid_nchem; /* cells with Nchem < 1 */
+    long long nonfinite;     /* cells with non-finite density or temperature */
+    int max_nchem;           /* Reaction::PushNchemMax (src/Reaction.cpp:953-957) */
+    long long stiff_cells;   /* cells with Nchem > 1 (queued when compacting) */
+    long long total_substeps;/* sum of Nchem over the interior */
+} eu3d_health_info;
+
+/* Per-kernel device time of the last call of each kind, from HIP events on
+ * the context stream (milliseconds); accumulated totals and call counts. */
+typedef struct {
+    double flux_ms, recover_ms, chem_ms, ghost_ms, cfl_ms; /* last call */
+    double flux_total_ms, recover_total_ms, chem_total_ms, ghost_total_ms, cfl_total_ms;
+    long long flux_calls, recover_calls, chem_calls, ghost_calls, cfl_calls;
+} eu3d_timing;
+
+/* ---- lifetime ---------------------------------------------------------- */
+/* Replaces the function-static device buffers of muscl_ausm_host /
+ * reaction_cells_host (src/FlowfieldCuda.cu:588-592, src/ReactionCuda.cu:155-159)
+ * and TimeAdv's device members (src/TimeAdv.cpp:222-231). */
+int eu3d_create(const eu3d_mesh *mesh, const eu3d_mechanism *mech, eu3d_ctx **out);
+int eu3d_destroy(eu3d_ctx *ctx);
+const char *eu3d_last_error(const eu3d_ctx *ctx);
+
+/* Host <-> device state.  Replaces the per-step uploads of muscl_ausm_host
+ * (src/FlowfieldCuda.cu:793-1009) and download_resident_output /
+ * reaction_packed_output (src/ReactionCuda.cu:2168-2263). */
+int eu3d_upload_state(eu3d_ctx *ctx, const eu3d_state *host);
+int eu3d_download_state(eu3d_ctx *ctx, eu3d_state *host);
+
+/* ---- the step ------------------------------------------------------------ */
+/* Flowfield::FieldBoundary_3dRSBI (src/Flowfield.cpp:593-807); device form of
+ * resident_muscl_x/_y/_z + resident_front_thermo (src/FlowfieldCuda.cu:627-696). */
+int eu3d_fill_ghosts(eu3d_ctx *ctx);
+
+/* Flowfield::CFLcondition (src/Flowfield.cpp:276-306): dt = cfl * min over the
+ * interior of min(dx/(|u|+c), dy/(|v|+c)), clamped to final_time - time. */
+int eu3d_cfl_dt(eu3d_ctx *ctx, double cfl, double time, double final_time, double *dt);
+
+/* Flowfield::Advection(EE, MUSCL_2) + the CS += RHS of Update_after_Adv:
+ * AUSM x3 (muscl_ausm_host, src/FlowfieldCuda.cu:793-1009), TimeAdv::EE
+ * (launch_ee_cuda, src/TimeAdvCuda.cu:54-82) and CS += RHS
+ * (src/Flowfield.cpp:1349-1354), fused.  Ghosts must be filled. */
+int eu3d_flux_update(eu3d_ctx *ctx, double dt);
+
+/* Flowfield::Update_after_CS() (src/Flowfield.cpp:1579-1617);
+ * post_advection_state_host (include/ReactionCuda.cuh). */
+int eu3d_recover_state(eu3d_ctx *ctx);
+
+/* TimeAdv::TVD_RK3 (src/TimeAdv.cpp:210-221, a no-op stub there): SSP-RK3,
+ * each stage = ghost fill + flux + stage combination + state recovery. */
+int eu3d_advance_rk3(eu3d_ctx *ctx, double dt);
+
+/* Euler::Trapezoid(type=1) chemistry after the advection update:
+ * Reaction::TrapezoidPrediction + per-cell adaptive trapezoid sub-steps with
+ * Explicit/Update_after_CS after each (src/Euler.cpp:461-493);
+ * reaction_cells_host (src/ReactionCuda.cu:2275-2627). flags: EU3D_CHEM_*. */
+int eu3d_chemistry(eu3d_ctx *ctx, double dt, int flags);
+
+/* One iteration of Euler::Computing's loop body (src/Euler.cpp:322-389)
+ * with the given dt: ghosts, advection (EE or RK3), recovery, chemistry. */
+int eu3d_step(eu3d_ctx *ctx, int scheme, double dt, int chem_flags);
+
+/* Euler::Computing (src/Euler.cpp:306-421): steps until final_time (within
+ * 1e-12*final_time) or max_steps.  cfl == 0 uses dt_fixed.  *time is advanced. */
+int eu3d_run(eu3d_ctx *ctx, int scheme, double cfl, double dt_fixed, double final_time, int max_steps,
+             int chem_flags, double *time, int *steps_done);
+
+/* Health counters of the last chemistry call; returns EU3D_ERR_HEALTH if
+ * invalid Nchem or non-finite cells were found. */
+int eu3d_health(eu3d_ctx *ctx, eu3d_health_info *out);
+
+int eu3d_synchronize(eu3d_ctx *ctx);
+int eu3d_timing_get(eu3d_ctx *ctx, eu3d_timing *out);
+int eu3d_timing_enable(eu3d_ctx *ctx, int on);
+
+/* ---- kernel-level probes (the reference's start-up self-checks) --------- */
+/* thermo_test (src/ReactionCuda.cu:1160-1200): cp, h, s of every species at n
+ * temperatures with gas constant R; outputs [n*ns], index t*ns+s. */
+int eu3d_probe_thermo(eu3d_ctx *ctx, const double *T, int n, double R, double *cp, double *h, double *s);
+
+/* Face fluxes of one direction (1=x, 2=y, 3=z) from the current device
+ * primitives, in the reference's F/G/Q layout [N*(ns+4)]; faces the
+ * reference's AUSM loop does not compute are 0 (src/Flowfield.cpp:1159-1163). */
+int eu3d_probe_face_flux(eu3d_ctx *ctx, int direction, double *flux);
+
+/* Reaction rates of n cells (reaction_rates_test, src/ReactionCuda.cu:1343-1420):
+ * T[n], Mc[n*ns] in mol/cm^3; outputs kf, kb, rr [n*nr], wi [n*ns]. */
+int eu3d_probe_rates(eu3d_ctx *ctx, const double *T, const double *Mc, int n, double *kf, double *kb,
+                     double *rr, double *wi);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

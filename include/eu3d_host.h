@@ -1,0 +1,52 @@
+/*
+ * eu3d_host.h -- C ABI of the host side (the reference's Euler driver with
+ * its Flowfield / Reaction / Mesh readers and initial conditions), for
+ * callers that start from the reference's input files (ctrl.txt, grid-*.txt,
+ * input-*.txt, mechanism, thermo.dat) instead of building the state
+ * themselves.  A solver owns one device context (include/eu3d.h); its host
+ * arrays use the reference layout and are refreshed by eu3d_solver_sync.
+ */
+#ifndef EU3D_HOST_H
+#define EU3D_HOST_H
+
+#include "eu3d.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct eu3d_solver eu3d_solver;
+
+/* Euler::Euler + FileRead + MeshProcess + InputRead + ReactionRead +
+ * FieldInitial (src/Euler.cpp:59-225); relative paths in the ctrl file are
+ * resolved against base_dir.  Creates the device context and uploads. */
+int eu3d_solver_open(const char *ctrl, const char *base_dir, eu3d_solver **out, char *err, int errlen);
+/* The same without touching the GPU: inputs read and the initial state built
+ * on the host; eu3d_solver_attach creates the device context and uploads. */
+int eu3d_solver_open_host(const char *ctrl, const char *base_dir, eu3d_solver **out, char *err, int errlen);
+int eu3d_solver_attach(eu3d_solver *s);
+int eu3d_solver_close(eu3d_solver *s);
+eu3d_ctx *eu3d_solver_ctx(eu3d_solver *s);
+/* dims[10]: ni nj nk bc NI NJ NK ns nr time_adv */
+int eu3d_solver_dims(const eu3d_solver *s, int *dims);
+/* info[6]: Final_Time cfl dt(ctrl) time last_dt iteration */
+int eu3d_solver_info(const eu3d_solver *s, double *info);
+/* Euler::Computing (src/Euler.cpp:306-421), at most max_steps iterations */
+int eu3d_solver_run(eu3d_solver *s, int max_steps, int chem_flags);
+/* Euler::SyncResidentOutput (src/Euler.cpp:1492-1534): device -> host arrays */
+int eu3d_solver_sync(eu3d_solver *s);
+/* host arrays -> device (after editing the host copy) */
+int eu3d_solver_upload(eu3d_solver *s);
+/* host arrays by reference name (U V W P D T C Ma Wav Rgas Cp H E Gamma Mc Mi
+ * Yi Di CMS CS xnode ynode znode coeff0 coeff1 mw ri stoi_f stoi_b react_tb af
+ * bf eaf); NULL if unknown */
+double *eu3d_solver_field(eu3d_solver *s, const char *name);
+int *eu3d_solver_nchem(eu3d_solver *s);
+/* EU3D_STATE_V1_LE dump (src/Euler.cpp:1539-1598) */
+int eu3d_solver_dump(eu3d_solver *s, const char *path, int num);
+const char *eu3d_solver_error(const eu3d_solver *s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

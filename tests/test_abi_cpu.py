@@ -1,0 +1,96 @@
+"""CPU checks of the product library: it loads, exports every entry point the
+headers declare, and its host-side readers and initial conditions reproduce
+the oracle's initial state bit for bit (no GPU needed)."""
+import os
+import re
+
+import numpy as np
+import pytest
+
+import cases
+import oracle
+from eu3d_load import eu3d, REPO
+
+
+def _declared(header):
+    text = open(os.path.join(REPO, "include", header)).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return set(re.findall(r"\b(eu3d_[a-z0-9_]+)\s*\(", text))
+
+
+def test_library_exports_every_declared_symbol():
+    L = eu3d.load()
+    names = _declared("eu3d.h") | _declared("eu3d_host.h")
+    assert len(names) >= 30
+    missing = [n for n in sorted(names) if not hasattr(L, n)]
+    assert not missing, missing
+    assert names <= set(eu3d.SIGNATURES), sorted(names - set(eu3d.SIGNATURES))
+
+
+def _bits(a):
+    return np.ascontiguousarray(a).view(np.int64)
+
+
+SCALARS = ["U", "V", "W", "P", "D", "T", "C", "Wav", "Rgas", "Cp", "H", "E", "Gamma"]
+SPECIES = ["Mc", "Yi", "Di"]
+
+
+@pytest.mark.parametrize("case", ["rsbi", "sbi", "znd"])
+def test_host_initial_state_matches_oracle(tmp_path, case):
+    ctrl = {"rsbi": lambda d: cases.rsbi(d), "sbi": lambda d: cases.sbi(d, ni=24, nj=12, nk=12),
+            "znd": lambda d: cases.znd(d, ni=101)}[case](str(tmp_path))
+    o = oracle.Oracle(ctrl)
+    s = eu3d.Solver(ctrl, device=False)
+    assert (s.ni, s.nj, s.nk, s.bc, s.ns) == (o.ni, o.nj, o.nk, o.bc, o.ns)
+    for n in ["xnode", "ynode", "znode", "coeff0", "coeff1", "mw", "ri"]:
+        assert np.array_equal(_bits(s.field(n)), _bits(o.field(n))), n
+    for n in SCALARS + SPECIES + ["CS"]:
+        a, b = o.interior(o.field(n)), s.interior(s.field(n))
+        assert np.array_equal(_bits(a), _bits(b)), n
+    # ghost points carry P, T, U and Yi from the initial condition
+    for n in ["P", "T", "U", "Yi"]:
+        assert np.array_equal(_bits(o.field(n)), _bits(s.field(n))), n
+    s.close()
+
+
+def test_mechanism_tables_match_oracle(tmp_path):
+    ctrl = cases.rsbi(str(tmp_path))
+    o = oracle.Oracle(ctrl)
+    s = eu3d.Solver(ctrl, device=False)
+    for n in ["af", "bf", "eaf"]:
+        assert np.array_equal(s.field(n), o.field_raw(n, None)), n
+    nr = o.nr
+    for n in ["stoi_f", "stoi_b", "react_tb"]:
+        ref = np.ctypeslib.as_array(o.L.or_field(o.h, n.encode()), shape=(o.ns * nr,))
+        assert np.array_equal(s.field(n).ravel(), ref), n
+    s.close()
+
+
+def test_bad_inputs_are_errors_not_exits(tmp_path):
+    ctrl = cases.write_case(str(tmp_path), "bad", 2, 5, 5, (0, 1, 0, 1, 0, 1),
+                            cases.RSBI_INPUT.format(T_pre=1000.0, M_wave=3.0))
+    with pytest.raises(eu3d.EU3DError):
+        eu3d.Solver(ctrl, device=False)
+    with pytest.raises(eu3d.EU3DError):
+        eu3d.Solver(str(tmp_path / "nope.txt"), device=False)
+
+
+def test_dump_format_and_comparer(tmp_path):
+    """The oracle's EU3D_STATE_V1_LE dump parses with the build's comparer,
+    matches itself, and a perturbation beyond atol+rtol is reported."""
+    from eu3d_load import compare_states
+    ctrl = cases.rsbi(str(tmp_path), steps=2)
+    o = oracle.Oracle(ctrl)
+    o.run(2)
+    a = str(tmp_path / "a.bin")
+    assert o.write_dump(a) == 0
+    meta, times, names, data = compare_states.read(a)
+    assert meta[:7] == [9, 17, 13, 4, 17, 13, 4] and meta[15] == 2
+    assert data.shape == (17 * 13 * 4, 17 + 5 * 9 + 4 + 1)
+    assert compare_states.compare(a, [a], out=open(os.devnull, "w")) == 0
+    raw = bytearray(open(a, "rb").read())
+    body = np.frombuffer(raw, dtype="<f8", offset=len(raw) - data.size * 8).copy()
+    body[names.index("T")] *= 1 + 1e-9
+    b = str(tmp_path / "b.bin")
+    open(b, "wb").write(bytes(raw[:len(raw) - data.size * 8]) + body.tobytes())
+    assert compare_states.compare(a, [b], out=open(os.devnull, "w")) == 1

@@ -1,0 +1,220 @@
+"""GPU parity: the HIP path (through the C ABI) against the CPU oracle on the
+same inputs.
+
+Bars (rel L-infinity = max|gpu - oracle| / max|oracle| per field, interior):
+  ghost/halo index maps                          bit-exact
+  conserved state after 1 step                   <= 1e-12
+  conserved state after 100 steps                <= 1e-8
+  Nchem                                          exact
+  compacted vs uncompacted chemistry             bit-exact
+"""
+import ctypes
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import cases
+import oracle
+from eu3d_load import eu3d, compare_states
+
+pytestmark = pytest.mark.gpu
+
+SCALARS = ["U", "V", "W", "P", "D", "T", "E", "H", "Gamma", "C"]
+
+
+def rel_linf(a, b):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    den = np.max(np.abs(b))
+    err = np.max(np.abs(a - b))
+    if den == 0:
+        return err
+    return err / den
+
+
+def note(msg):
+    sys.stdout.write(msg + "\n")
+    sys.stdout.flush()
+
+
+def pair(ctrl):
+    o = oracle.Oracle(ctrl)
+    s = eu3d.Solver(ctrl)
+    return o, s
+
+
+def compare_state(o, s, tol, what, fields=SCALARS, species=("Yi",), cs=True):
+    worst = {}
+    for n in fields:
+        worst[n] = rel_linf(s.interior(s.field(n)), o.interior(o.field(n)))
+    for n in species:
+        a, b = s.interior(s.field(n)), o.interior(o.field(n))
+        for q in range(o.ns):
+            worst[f"{n}{q}"] = rel_linf(a[..., q], b[..., q])
+    if cs:
+        a, b = s.interior(s.field("CS")), o.interior(o.field("CS"))
+        for q in range(o.ns + 4):
+            worst[f"CS{q}"] = rel_linf(a[..., q], b[..., q])
+    bad = {k: v for k, v in worst.items() if not (v <= tol)}
+    note(f"{what}: max rel Linf = {max(worst.values()):.3e} ({max(worst, key=worst.get)})")
+    assert not bad, f"{what}: {bad}"
+    return worst
+
+
+def test_ghost_fill_index_maps_bitexact(tmp_path):
+    """FieldBoundary_3dRSBI: every mirrored copy is bit-exact; the y-front
+    thermo recomputation (D, H, E, Gamma) is within 1e-14."""
+    note("ghost fill: rsbi 17x13x4")
+    o, s = pair(cases.rsbi(str(tmp_path)))
+    L = eu3d.load()
+    eu3d.check(L.eu3d_fill_ghosts(s.ctx), s.ctx, "fill_ghosts")
+    s.sync()
+    o.field_boundary()
+    for n in ["P", "U", "V", "W", "T", "Yi", "D"]:
+        a, b = s.field(n), o.field(n)
+        assert np.array_equal(a.view(np.int64), b.view(np.int64)), n
+    for n in ["H", "E", "Gamma"]:
+        assert rel_linf(s.field(n), o.field(n)) <= 1e-14, n
+
+
+def test_thermo_probe(tmp_path):
+    """NASA-9 cp/h/s at the reference's probe temperatures (src/Reaction.cpp:333-335) and a sweep."""
+    o, s = pair(cases.rsbi(str(tmp_path)))
+    L = eu3d.load()
+    T = np.array([300.0, 999.0, 1000.0, 1001.0, 2000.0] + list(np.linspace(250.0, 4000.0, 41)))
+    n, ns = len(T), o.ns
+    cp, h, si = np.zeros(n * ns), np.zeros(n * ns), np.zeros(n * ns)
+    R = 8.31434
+    eu3d.check(L.eu3d_probe_thermo(s.ctx, eu3d.ptr(T), n, R, eu3d.ptr(cp), eu3d.ptr(h), eu3d.ptr(si)), s.ctx)
+    c0, c1 = o.field("coeff0"), o.field("coeff1")
+    OL = oracle.lib()
+    for t in range(n):
+        for q in range(ns):
+            ref = [OL.or_getcpi(T[t], R, q, oracle.ptr(c0), oracle.ptr(c1), ns),
+                   OL.or_gethi(T[t], R, q, oracle.ptr(c0), oracle.ptr(c1), ns),
+                   OL.or_getsi(T[t], R, q, oracle.ptr(c0), oracle.ptr(c1), ns)]
+            got = [cp[t * ns + q], h[t * ns + q], si[t * ns + q]]
+            for a, b in zip(got, ref):
+                assert abs(a - b) <= 1e-13 * (1.0 + abs(b)), (T[t], q, a, b)
+
+
+@pytest.mark.parametrize("direction", [1, 2, 3])
+def test_face_flux_matches_oracle(tmp_path, direction):
+    """MUSCL_2 + AUSM face fluxes of one direction from the same ghost-filled state."""
+    note(f"face flux dir {direction}")
+    o, s = pair(cases.rsbi(str(tmp_path)))
+    L = eu3d.load()
+    eu3d.check(L.eu3d_fill_ghosts(s.ctx), s.ctx)
+    o.field_boundary()
+    o.set_dt(3e-8)
+    o.advection()
+    ref = o.field({1: "F", 2: "G", 3: "Q"}[direction])
+    got = np.zeros_like(ref)
+    eu3d.check(L.eu3d_probe_face_flux(s.ctx, direction, eu3d.ptr(got)), s.ctx)
+    b = o.bc
+    sl = (slice(b - 1, b + o.ni), slice(b - 1, b + o.nj), slice(b - 1, b + o.nk))
+    for q in range(o.ns + 4):
+        assert rel_linf(got[sl][..., q], ref[sl][..., q]) <= 1e-13, q
+
+
+def test_ee_advection_one_step(tmp_path):
+    """Flux + EE + CS += RHS + Update_after_CS, before chemistry."""
+    note("ee advection: rsbi 17x13x4")
+    o, s = pair(cases.rsbi(str(tmp_path)))
+    L = eu3d.load()
+    dt = 3e-8
+    for rc in (L.eu3d_fill_ghosts(s.ctx), L.eu3d_flux_update(s.ctx, dt), L.eu3d_recover_state(s.ctx)):
+        eu3d.check(rc, s.ctx)
+    s.sync()
+    o.set_dt(dt)
+    o.field_boundary()
+    o.advection()
+    o.update_after_adv()
+    compare_state(o, s, 1e-12, "EE advection, 1 step")
+
+
+@pytest.mark.parametrize("steps,tol", [(1, 1e-12), (100, 1e-8)])
+def test_rsbi_ee_full_steps(tmp_path, steps, tol):
+    """The reference's own configuration (3dRSBI, EE, Trapezoid-Adaptive, dt 3e-8)
+    on its 17x13x4 validation grid."""
+    note(f"rsbi ee {steps} steps")
+    o, s = pair(cases.rsbi(str(tmp_path), steps=steps))
+    assert o.run(steps) == steps
+    s.run(steps)
+    s.sync()
+    compare_state(o, s, tol, f"RSBI EE {steps} steps")
+    assert np.array_equal(s.interior(s.nchem()), o.interior(o.nchem()))
+    rc, h = s.health()
+    assert rc == 0 and h.invalid_nchem == 0 and h.nonfinite == 0
+
+
+def test_rsbi_stiff_chemistry(tmp_path):
+    """Hot RSBI with a large step: cells with Nchem > 1 go through the compacted queue."""
+    note("rsbi stiff 10 steps")
+    ctrl = cases.rsbi(str(tmp_path), steps=10, T_pre=1200.0, dt=3e-7)
+    o, s = pair(ctrl)
+    o.run(10)
+    s.run(10)
+    s.sync()
+    n_o, n_s = o.interior(o.nchem()), s.interior(s.nchem())
+    assert n_o.max() > 1
+    assert np.array_equal(n_s, n_o)
+    compare_state(o, s, 1e-10, "RSBI stiff 10 steps")
+
+
+def test_compacted_equals_uncompacted(tmp_path):
+    ctrl = cases.rsbi(str(tmp_path), steps=10, T_pre=1200.0, dt=3e-7)
+    a = eu3d.Solver(ctrl)
+    b = eu3d.Solver(ctrl)
+    a.run(10, eu3d.CHEM_COMPACT | eu3d.CHEM_KEEP_CMS)
+    b.run(10, eu3d.CHEM_KEEP_CMS)
+    a.sync()
+    b.sync()
+    assert a.interior(a.nchem()).max() > 1
+    for n in ["CS", "T", "E", "P", "D", "U", "Yi", "CMS", "H", "Gamma"]:
+        assert np.array_equal(a.field(n).view(np.int64), b.field(n).view(np.int64)), n
+    assert np.array_equal(a.nchem(), b.nchem())
+
+
+def test_sbi_rk3_cfl(tmp_path):
+    """Spherical shock-bubble, SSP-RK3, CFL 0.5: dt sequence and state."""
+    note("sbi rk3 20 steps")
+    ctrl = cases.sbi(str(tmp_path), ni=32, nj=16, nk=16)
+    o, s = pair(ctrl)
+    for step in range(20):
+        dt_o = o.cfl(0.5, 1.0)
+        o.step(dt_o)
+        s.run(1)
+        dt_s = s.info()["last_dt"]
+        assert abs(dt_s - dt_o) <= 1e-12 * dt_o, (step, dt_s, dt_o)
+        if step == 0:
+            s.sync()
+            compare_state(o, s, 1e-12, "SBI RK3 1 step")
+    s.sync()
+    compare_state(o, s, 1e-8, "SBI RK3 20 steps")
+
+
+def test_znd_rk3(tmp_path):
+    note("znd rk3 30 steps")
+    ctrl = cases.znd(str(tmp_path), ni=201)
+    o, s = pair(ctrl)
+    for step in range(30):
+        o.step(o.cfl(0.5, 1.0))
+    s.run(30)
+    s.sync()
+    compare_state(o, s, 1e-8, "ZND RK3 30 steps")
+
+
+def test_state_dump_v1(tmp_path):
+    """EU3D_STATE_V1_LE dumps of the GPU run and the oracle agree under the
+    reference comparer's semantics (atol 1e-12 + rtol 1e-10, Nchem exact)."""
+    ctrl = cases.rsbi(str(tmp_path), steps=3)
+    o, s = pair(ctrl)
+    o.run(3)
+    s.run(3)
+    pa, pb = str(tmp_path / "gpu.bin"), str(tmp_path / "cpu.bin")
+    s.dump(pa)
+    assert o.write_dump(pb) == 0
+    assert compare_states.compare(pb, [pa]) == 0
