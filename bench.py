@@ -1,0 +1,192 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X reactive-Euler step.
+
+Workload: the 3-D reactive shock-bubble (256 x 128 x 128 nodes, 9 species /
+19 reactions H2 mechanism, SSP-RK3 + MUSCL/AUSM+ advection, adaptive
+trapezoid chemistry with stiff-cell compaction, CFL 0.5), plain FP64 path,
+synthetic initial condition (no data files besides the mechanism/thermo
+tables).  A step is one full iteration of the time loop: CFL reduction,
+three RK stages (ghost fill + fused flux/update + state recovery each) and
+the chemistry.  Prints one JSON line.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, REPO)
+
+METRIC = "cell-steps/s (flux+chem) 3D H2 reactive Euler; MLP-chem vs FP64 speedup"
+HBM_PEAK_GBS = 8000.0     # MI355X HBM3E
+FP64_PEAK_TFLOPS = 78.6   # MI355X FP64 (vector and matrix)
+
+# Algorithmic HBM bytes of the fused flux/update kernel per interior node and
+# RK stage: read 16 primitive fields + 13 conserved, write 13 conserved
+# (42 doubles).  RK stages 2 and 3 also read the stage-0 copy (13 doubles)
+# and stage 1 writes it: +104 B each.  Ghost/halo re-reads are overhead.
+FLUX_BYTES_PER_NODE = 336
+RK_EXTRA_BYTES_PER_NODE = 104
+
+
+def eu3d_pkg():
+    import importlib
+    return importlib.import_module("eu3d-gpu_amd")
+
+
+def chem_flops_per_substep(s):
+    """Algorithmic FP64 operations of one trapezoid sub-step (rates, update,
+    state rebuild with two Newton iterations), counted from the mechanism's
+    non-zero table entries (add/sub/mul/div/sqrt = 1; exp/log/pow separately)."""
+    ns, nr = s.ns, s.nr
+    nf, nb, tb = s.field("stoi_f"), s.field("stoi_b"), s.field("react_tb")
+    bf, eaf = s.field("bf"), s.field("eaf")
+    fl, tr = 0, 0
+    # thermo h, s, g for every species: h 21, s 20, g 2; shared 1/T, T^2 and
+    # T^3..T^5, T^-2, log T
+    fl += ns * 43 + 2
+    tr += 5
+    for r in range(nr):
+        fl += 4                                   # A*T^b*exp(-E/(Ru T))
+        tr += (bf[r] not in (0.0, 1.0)) + (eaf[r] != 0.0)
+        for q in range(ns):
+            d = nb[q, r] - nf[q, r]
+            fl += 2 * (d != 0)                    # delta_g
+            fl += (nf[q, r] != 0) * int(nf[q, r]) + (nb[q, r] != 0) * int(nb[q, r])
+            fl += 2 * (tb[q, r] != 0)             # third-body sum
+            fl += 3 * (d != 0)                    # net production
+            fl += 14 * (nb[q, r] != 0 or nf[q, r] != 0)  # P and Q sums
+        fl += 3 + 2 + 1 + 3                       # Kp, Kc, kb, RR
+        tr += 1                                   # exp in Kp
+    fl += ns * (1 + 1 + 1 + 9 + 2)                # Mw factors, P/Mc, update, unit scaling
+    fl += ns + 3 + 8                              # Explicit: density, momenta, energy
+    newton = ns * 38 + 8                          # cp and h sums + residual per iteration
+    fl += 2 * ns + 2 * newton + 3 + 7 + 2 * ns + 1 + 2 + 2 + 2 + ns * 15 + 4 + 3
+    tr += 2 * 5
+    return int(fl), int(tr)
+
+
+def cpu_baseline(ctrl, base, threads):
+    """The oracle (CPU restatement) on the same workload: one step from the
+    initial state over the full grid, OpenMP across `threads` host cores."""
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import oracle
+    o = oracle.Oracle(ctrl, base_dir=base, omp=True, threads=threads)
+    used = oracle.lib(True).or_num_threads(0)
+    n = o.ni * o.nj * o.nk
+    t0 = time.perf_counter()
+    dt = o.cfl(0.5, o.info()["final_time"])
+    o.step(dt)
+    el = time.perf_counter() - t0
+    del o
+    return {"value": n / el, "unit": "cell-steps/s", "cores": int(used), "kind": "port",
+            "sample": f"one SSP-RK3 + chemistry step from the initial state, full {n}-node grid, "
+                      f"oracle/liboracle_omp.so with {used} OpenMP threads, {el:.2f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--ctrl", default=None, help="ctrl file (default: input/ctrl-3dSBI.txt of the package)")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(multi-GPU decomposition is out of scope)", file=sys.stderr)
+        sys.exit(2)
+
+    eu3d = eu3d_pkg()
+    L = eu3d.load()
+    base = os.path.join(REPO, "eu3d-gpu_amd")
+    ctrl = args.ctrl or os.path.join(base, "input", "ctrl-3dSBI.txt")
+    s = eu3d.Solver(ctrl, base_dir=base)
+    ctx = s.ctx
+    n_int = s.ni * s.nj * s.nk
+    flags = eu3d.CHEM_COMPACT | eu3d.CHEM_KEEP_CMS
+    print(f"bench: {s.ni}x{s.nj}x{s.nk} nodes, warmup {args.warmup}, steps {args.steps}", file=sys.stderr,
+          flush=True)
+    s.run(args.warmup, flags)
+    eu3d.check(L.eu3d_synchronize(ctx), ctx)
+
+    # timed region: HIP events around every kernel on the context stream
+    eu3d.check(L.eu3d_timing_enable(ctx, 1), ctx)
+    tm = eu3d.Timing()
+    eu3d.check(L.eu3d_timing_get(ctx, tm), ctx)  # resets the accumulators
+    eu3d.check(L.eu3d_synchronize(ctx), ctx)
+    t0 = time.perf_counter()
+    s.run(args.steps, flags)
+    eu3d.check(L.eu3d_synchronize(ctx), ctx)
+    el = time.perf_counter() - t0
+    eu3d.check(L.eu3d_timing_get(ctx, tm), ctx)
+    rc, h = s.health()
+    if rc != 0:
+        print(f"bench: health check failed: invalid_nchem={h.invalid_nchem} nonfinite={h.nonfinite}",
+              file=sys.stderr)
+        sys.exit(3)
+    info = s.info()
+    value = n_int * args.steps / el
+
+    # flux/update roofline (HBM)
+    flux_avg_ms = tm.flux_total_ms / max(tm.flux_calls, 1)
+    flux_bytes = FLUX_BYTES_PER_NODE * n_int + (2 * RK_EXTRA_BYTES_PER_NODE * n_int) / 3.0
+    flux_gbs = flux_bytes / (flux_avg_ms * 1e-3) / 1e9
+    # chemistry roofline (FP64)
+    fl, tr = chem_flops_per_substep(s)
+    chem_avg_ms = tm.chem_total_ms / max(tm.chem_calls, 1)
+    substeps = h.total_substeps  # of the last chemistry call
+    chem_tflops = fl * substeps / (chem_avg_ms * 1e-3) / 1e12
+    kern = {"flux_update": tm.flux_total_ms, "recover": tm.recover_total_ms, "chemistry": tm.chem_total_ms,
+            "ghost": tm.ghost_total_ms, "cfl": tm.cfl_total_ms}
+    dominant = max(kern, key=kern.get)
+    roof_flux = {"bound": "hbm", "achieved": round(flux_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                 "frac": round(flux_gbs / HBM_PEAK_GBS, 4), "traffic": None,
+                 "kernel": "k_flux_update", "avg_ms": round(flux_avg_ms, 4),
+                 "bytes_per_launch": int(flux_bytes)}
+    roof_chem = {"bound": "mfma", "achieved": round(chem_tflops, 3), "peak": FP64_PEAK_TFLOPS,
+                 "unit": "TFLOP/s", "frac": round(chem_tflops / FP64_PEAK_TFLOPS, 4), "traffic": None,
+                 "kernel": "k_chem_first+k_chem_queue", "avg_ms": round(chem_avg_ms, 4),
+                 "flops_per_substep": fl, "transcendentals_per_substep": tr, "substeps": int(substeps)}
+    roofline = roof_chem if dominant == "chemistry" else roof_flux
+    profile_traffic = os.path.join(REPO, "profiles", "traffic.json")
+    if os.path.exists(profile_traffic):
+        with open(profile_traffic) as f:
+            tr_json = json.load(f)
+        roof_flux["traffic"] = tr_json.get("k_flux_update")
+        roof_chem["traffic"] = tr_json.get("k_chem_first")
+
+    out = {
+        "metric": METRIC, "value": value, "unit": "cell-steps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": el / args.steps * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "3D reactive shock-bubble, plain FP64 path (configs[1])",
+                   "grid": [s.ni, s.nj, s.nk], "species": s.ns, "reactions": s.nr,
+                   "scheme": "SSP-RK3 + MUSCL(minmod) + AUSM+", "chemistry": "adaptive trapezoid, compacted",
+                   "cfl": 0.5, "sim_time_end": info["time"]},
+        "roofline": roofline,
+        "roofline_flux": roof_flux,
+        "roofline_chem": roof_chem,
+        "kernel_ms_total": {k: round(v, 3) for k, v in kern.items()},
+        "chem_health": {"max_nchem": h.max_nchem, "stiff_cells": h.stiff_cells,
+                        "substeps_last_call": h.total_substeps},
+        "surrogate_speedup": None,
+    }
+    s.close()
+    if not args.no_cpu_baseline:
+        threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
+        print(f"bench: cpu baseline with {threads} threads", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline(ctrl, base, threads)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

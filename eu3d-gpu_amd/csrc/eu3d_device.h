@@ -86,27 +86,43 @@ __device__ __forceinline__ TPow tpow(double T)
     return t;
 }
 
-// Reaction::GetCpi / GetHi / GetSi (src/Reaction.cpp:1165-1220), coefficient
-// block a[k*ns] for one species (stride ns).
-__device__ __forceinline__ double getcpi(const TPow &t, double R, const double *a, int ns)
+// NASA-9 coefficients of one temperature range.  Both ranges are read with
+// uniform (scalar) loads and selected per lane, so no lane-dependent address
+// is formed for the table.
+struct CoefSel {
+    const double *c0, *c1;
+    bool low; // T < 1000 K (include/ThermoConstants.hpp:3)
+};
+
+__device__ __forceinline__ CoefSel coeffs(const DevMech &m, double T)
 {
-    return R * (a[0] * t.Tm2 + a[ns] * t.Tm1 + a[2 * ns] + a[3 * ns] * t.T + a[4 * ns] * t.T2 +
-                a[5 * ns] * t.T3 + a[6 * ns] * t.T4);
-}
-__device__ __forceinline__ double gethi(const TPow &t, double R, const double *a, int ns)
-{
-    return R * (-a[0] * t.Tm1 + a[ns] * t.lnT + a[2 * ns] * t.T + a[3 * ns] * t.T2 / 2 + a[4 * ns] * t.T3 / 3 +
-                a[5 * ns] * t.T4 / 4 + a[6 * ns] * t.T5 / 5 + a[7 * ns]);
-}
-__device__ __forceinline__ double getsi(const TPow &t, double R, const double *a, int ns)
-{
-    return R * (-a[0] * t.Tm2 / 2 - a[ns] * t.Tm1 + a[2 * ns] * t.lnT + a[3 * ns] * t.T + a[4 * ns] * t.T2 / 2 +
-                a[5 * ns] * t.T3 / 3 + a[6 * ns] * t.T4 / 4 + a[8 * ns]);
+    return CoefSel{m.c0, m.c1, T < EU3D_THERMO_SWITCH_T};
 }
 
-__device__ __forceinline__ const double *coeffs(const DevMech &m, double T, int s)
+__device__ __forceinline__ double cf(const CoefSel &c, int idx)
 {
-    return (T < EU3D_THERMO_SWITCH_T ? m.c0 : m.c1) + s;
+    double a = c.c0[idx], b = c.c1[idx];
+    return c.low ? a : b;
+}
+
+// Reaction::GetCpi / GetHi / GetSi (src/Reaction.cpp:1165-1220) of species s
+// (coefficient k of species s at index k*ns + s).
+__device__ __forceinline__ double getcpi(const TPow &t, double R, const CoefSel &c, int s, int ns)
+{
+    return R * (cf(c, s) * t.Tm2 + cf(c, ns + s) * t.Tm1 + cf(c, 2 * ns + s) + cf(c, 3 * ns + s) * t.T +
+                cf(c, 4 * ns + s) * t.T2 + cf(c, 5 * ns + s) * t.T3 + cf(c, 6 * ns + s) * t.T4);
+}
+__device__ __forceinline__ double gethi(const TPow &t, double R, const CoefSel &c, int s, int ns)
+{
+    return R * (-cf(c, s) * t.Tm1 + cf(c, ns + s) * t.lnT + cf(c, 2 * ns + s) * t.T + cf(c, 3 * ns + s) * t.T2 / 2 +
+                cf(c, 4 * ns + s) * t.T3 / 3 + cf(c, 5 * ns + s) * t.T4 / 4 + cf(c, 6 * ns + s) * t.T5 / 5 +
+                cf(c, 7 * ns + s));
+}
+__device__ __forceinline__ double getsi(const TPow &t, double R, const CoefSel &c, int s, int ns)
+{
+    return R * (-cf(c, s) * t.Tm2 / 2 - cf(c, ns + s) * t.Tm1 + cf(c, 2 * ns + s) * t.lnT + cf(c, 3 * ns + s) * t.T +
+                cf(c, 4 * ns + s) * t.T2 / 2 + cf(c, 5 * ns + s) * t.T3 / 3 + cf(c, 6 * ns + s) * t.T4 / 4 +
+                cf(c, 8 * ns + s));
 }
 
 // ---------------------------------------------------------------------------
@@ -126,12 +142,12 @@ __device__ __forceinline__ double get_temp(const DevMech &m, double T, double E,
     while (count < 10) {
         T0 = T_temp;
         TPow t = tpow(T0);
+        const CoefSel a = coeffs(m, T0);
         double sh = 0.0, scp = 0.0;
 #pragma unroll
         for (int s = 0; s < NS; s++) {
-            const double *a = coeffs(m, T0, s);
-            double cp = getcpi(t, m.ri[s], a, NS);
-            double h = gethi(t, m.ri[s], a, NS);
+            double cp = getcpi(t, m.ri[s], a, s, NS);
+            double h = gethi(t, m.ri[s], a, s, NS);
             sh += Y[s] * h;
             scp += Y[s] * cp;
         }
@@ -178,10 +194,11 @@ __device__ __forceinline__ void recover(const DevMech &m, const double (&cs)[NS 
     double P = T * D * Rgas;
     double H = P / D + E;
     TPow t = tpow(T);
+    const CoefSel a = coeffs(m, T);
     double scp = 0.0;
 #pragma unroll
     for (int s = 0; s < NS; s++)
-        scp += getcpi(t, m.ri[s], coeffs(m, T, s), NS) * o.Y[s];
+        scp += getcpi(t, m.ri[s], a, s, NS) * o.Y[s];
     double Cp = scp * 1000;
     double G = Cp / (Cp - R * sYM * 1000);
     double C = sqrt(G * P / D);
@@ -352,11 +369,11 @@ __device__ __forceinline__ void rates(const DevMech &m, double T, const double (
     double Gi[NS];
     {
         TPow t = tpow(T);
+        const CoefSel a = coeffs(m, T);
 #pragma unroll
         for (int s = 0; s < NS; s++) {
-            const double *a = coeffs(m, T, s);
-            double Hi = gethi(t, R, a, NS);
-            double Si = getsi(t, R, a, NS);
+            double Hi = gethi(t, R, a, s, NS);
+            double Si = getsi(t, R, a, s, NS);
             Gi[s] = Hi - Si * T;
         }
     }
@@ -368,6 +385,7 @@ __device__ __forceinline__ void rates(const DevMech &m, double T, const double (
             o.P[s] = 0.0;
         }
     }
+#pragma unroll 1
     for (int r = 0; r < nr; r++) {
         double KF = m.af[r] * pow_small(T, m.bf[r]);
         if (m.eaf[r] != 0.0)

@@ -230,15 +230,15 @@ __global__ void __launch_bounds__(kBlock)
         prim[(7 + q) * N + d] = Y[q];
     }
     TPow tp = tpow(Tt);
+    const CoefSel a = coeffs(m, Tt);
     double sR = 0.0, sCp = 0.0, sH = 0.0, sE = 0.0;
 #pragma unroll
     for (int q = 0; q < NS; q++)
         sR += Y[q] * m.ri[q];
 #pragma unroll
     for (int q = 0; q < NS; q++) {
-        const double *a = coeffs(m, Tt, q);
-        double cp = getcpi(tp, m.ri[q], a, NS);
-        double h = gethi(tp, m.ri[q], a, NS);
+        double cp = getcpi(tp, m.ri[q], a, q, NS);
+        double h = gethi(tp, m.ri[q], a, q, NS);
         double e = h - m.ri[q] * Tt;
         sCp += cp * Y[q];
         sH += Y[q] * h;
@@ -278,65 +278,116 @@ __global__ void __launch_bounds__(kBlock) k_ghost_z(Grid g, int nprim, double *p
 }
 
 // ---------------------------------------------------------------------------
-// fused flux + conservative update.  MODE 0: EE (CS += RHS); 1..3: SSP-RK3
+// Fused flux + conservative update.  MODE 0: EE (CS += RHS); 1..3: SSP-RK3
 // stages (1 also saves CS into CS0).
+//
+// A block owns kTJ rows (j) of kTK = 64 cells (k, one wave per row) and
+// marches over kMarch planes (i).  Every face flux is computed once:
+//   x faces: by the owning thread, the plus face of plane i is kept in
+//            registers as the minus face of plane i+1;
+//   y faces: kTJ+1 rows of 64 faces, one per thread (the extra wave computes
+//            the faces below the tile), exchanged through LDS;
+//   z faces: kTJ rows of 65 faces on the first kTJ*65 threads, through LDS.
+// RHS = -dt/dx (F+ - F-) - dt/dy (G+ - G-) - dt/dz (Q+ - Q-) in that order
+// (TimeAdv::EE, src/TimeAdv.cpp:51-54).
 // ---------------------------------------------------------------------------
+constexpr int kTJ = 4;
+constexpr int kTK = 64;
+constexpr int kFluxThreads = 64 * (kTJ + 1);
+constexpr int kMarch = 16;
+#ifndef EU3D_FLUX_WAVES_PER_EU
+#define EU3D_FLUX_WAVES_PER_EU 2
+#endif
+
 template <int NS, int MODE>
-__global__ void __launch_bounds__(kBlock)
-    k_flux_update(Grid g, const double *__restrict__ prim, double *__restrict__ CS, double *__restrict__ CS0,
-                  const double *__restrict__ xn, const double *__restrict__ yn, const double *__restrict__ zn,
-                  double dt)
+__global__ void __launch_bounds__(kFluxThreads) __attribute__((amdgpu_waves_per_eu(EU3D_FLUX_WAVES_PER_EU)))
+    k_flux_tile(Grid g, const double *__restrict__ prim, double *__restrict__ CS, double *__restrict__ CS0,
+                const double *__restrict__ xn, const double *__restrict__ yn, const double *__restrict__ zn,
+                double dt)
 {
     constexpr int NV = NS + 4;
-    long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-    int i, j, k;
-    if (!interior_cell(g, t, i, j, k))
-        return;
+    constexpr int NYF = (kTJ + 1) * kTK; // y faces per plane
+    constexpr int NZF = kTJ * (kTK + 1); // z faces per plane
+    static_assert(NZF <= NYF, "LDS face buffer sized for the y faces");
+    __shared__ double face[NV * NYF];
+    const int t = threadIdx.x;
     const long long N = g.N;
-    const long long c = gidx(g, i, j, k);
-    double rhs[NV], Fm[NV], Fp[NV];
-    // x
-    face_flux<NS>(0, prim, N, c - g.sI, g.sI, Fm);
-    face_flux<NS>(0, prim, N, c, g.sI, Fp);
-    {
-        double cx = -dt / (xn[i] - xn[i - 1]);
+    const int k0 = g.bc + blockIdx.x * kTK;
+    const int j0 = g.bc + blockIdx.y * kTJ;
+    const int i0 = g.bc + blockIdx.z * kMarch;
+    const int i1 = min(i0 + kMarch, g.ni + g.bc);
+    const int kend = g.nk + g.bc, jend = g.nj + g.bc;
+    // the cell this thread owns
+    const int r = t >> 6, l = t & 63;
+    const int jc = j0 + r, kc = k0 + l;
+    const bool own = (t < kTJ * kTK) && (jc < jend) && (kc < kend);
+    // the y face this thread computes: between j = yj and yj+1 at k = yk
+    const int yj = j0 - 1 + (t >> 6), yk = k0 + (t & 63);
+    const bool yact = (yj < jend) && (yk < kend);
+    // the z face this thread computes: between k = zk and zk+1 in row zj
+    const int zr = t / (kTK + 1), zf = t - zr * (kTK + 1);
+    const int zj = j0 + zr, zk = k0 - 1 + zf;
+    const bool zact = (t < NZF) && (zj < jend) && (zk < kend);
+
+    double rhs[NV], Fxm[NV];
+    if (own)
+        face_flux<NS>(0, prim, N, gidx(g, i0 - 1, jc, kc), g.sI, Fxm);
+    for (int i = i0; i < i1; i++) {
+        if (own) {
+            double Fxp[NV];
+            face_flux<NS>(0, prim, N, gidx(g, i, jc, kc), g.sI, Fxp);
+            double cx = -dt / (xn[i] - xn[i - 1]);
 #pragma unroll
-        for (int q = 0; q < NV; q++)
-            rhs[q] = cx * (Fp[q] - Fm[q]);
-    }
-    // y
-    face_flux<NS>(1, prim, N, c - g.sJ, g.sJ, Fm);
-    face_flux<NS>(1, prim, N, c, g.sJ, Fp);
-    {
-        double cy = dt / (yn[j] - yn[j - 1]);
-#pragma unroll
-        for (int q = 0; q < NV; q++)
-            rhs[q] = rhs[q] - cy * (Fp[q] - Fm[q]);
-    }
-    // z
-    face_flux<NS>(2, prim, N, c - 1, 1, Fm);
-    face_flux<NS>(2, prim, N, c, 1, Fp);
-    {
-        double cz = dt / (zn[k] - zn[k - 1]);
-#pragma unroll
-        for (int q = 0; q < NV; q++)
-            rhs[q] = rhs[q] - cz * (Fp[q] - Fm[q]);
-    }
-#pragma unroll
-    for (int q = 0; q < NV; q++) {
-        double u = CS[q * N + c];
-        double v;
-        if (MODE == 0) {
-            v = u + rhs[q];
-        } else if (MODE == 1) {
-            CS0[q * N + c] = u;
-            v = u + rhs[q];
-        } else if (MODE == 2) {
-            v = 0.75 * CS0[q * N + c] + 0.25 * (u + rhs[q]);
-        } else {
-            v = (1.0 / 3.0) * CS0[q * N + c] + (2.0 / 3.0) * (u + rhs[q]);
+            for (int q = 0; q < NV; q++) {
+                rhs[q] = cx * (Fxp[q] - Fxm[q]);
+                Fxm[q] = Fxp[q];
+            }
         }
-        CS[q * N + c] = v;
+        if (yact) {
+            double F[NV];
+            face_flux<NS>(1, prim, N, gidx(g, i, yj, yk), g.sJ, F);
+#pragma unroll
+            for (int q = 0; q < NV; q++)
+                face[q * NYF + t] = F[q];
+        }
+        __syncthreads();
+        if (own) {
+            double cy = dt / (yn[jc] - yn[jc - 1]);
+#pragma unroll
+            for (int q = 0; q < NV; q++)
+                rhs[q] = rhs[q] - cy * (face[q * NYF + (r + 1) * kTK + l] - face[q * NYF + r * kTK + l]);
+        }
+        __syncthreads();
+        if (zact) {
+            double F[NV];
+            face_flux<NS>(2, prim, N, gidx(g, i, zj, zk), 1, F);
+#pragma unroll
+            for (int q = 0; q < NV; q++)
+                face[q * NZF + t] = F[q];
+        }
+        __syncthreads();
+        if (own) {
+            double cz = dt / (zn[kc] - zn[kc - 1]);
+            const long long c = gidx(g, i, jc, kc);
+#pragma unroll
+            for (int q = 0; q < NV; q++) {
+                double rq = rhs[q] - cz * (face[q * NZF + r * (kTK + 1) + l + 1] - face[q * NZF + r * (kTK + 1) + l]);
+                double u = CS[q * N + c];
+                double v;
+                if (MODE == 0) {
+                    v = u + rq;
+                } else if (MODE == 1) {
+                    CS0[q * N + c] = u;
+                    v = u + rq;
+                } else if (MODE == 2) {
+                    v = 0.75 * CS0[q * N + c] + 0.25 * (u + rq);
+                } else {
+                    v = (1.0 / 3.0) * CS0[q * N + c] + (2.0 / 3.0) * (u + rq);
+                }
+                CS[q * N + c] = v;
+            }
+        }
+        __syncthreads();
     }
 }
 
@@ -523,6 +574,10 @@ __device__ __forceinline__ void chem_store(long long N, long long c, const doubl
     store_state<NS>(N, c, st, prim, T, E, C);
 }
 
+#ifndef EU3D_CHEM_WAVES_PER_EU
+#define EU3D_CHEM_WAVES_PER_EU 2
+#endif
+
 __device__ __forceinline__ unsigned long long lanemask_lt()
 {
     unsigned int lane = threadIdx.x & 63;
@@ -534,7 +589,7 @@ __device__ __forceinline__ unsigned long long lanemask_lt()
 // reference evaluates them twice on identical inputs).  Cells with Nchem > 1
 // are appended to the stiff queue.
 template <int NS>
-__global__ void __launch_bounds__(kBlock)
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(EU3D_CHEM_WAVES_PER_EU)))
     k_chem_first(Grid g, const DevMech *__restrict__ mp, double dtm, double *__restrict__ CS,
                  double *__restrict__ prim, double *__restrict__ T, double *__restrict__ E, double *__restrict__ C,
                  double *__restrict__ CMS, int *__restrict__ nchem, int *__restrict__ queue, ChemCounters *cnt,
@@ -619,7 +674,7 @@ __device__ __forceinline__ void chem_rest(const Grid &g, const DevMech &m, long 
 
 // Compacted: waves fetch 64 queued cells at a time until the queue is drained.
 template <int NS>
-__global__ void __launch_bounds__(kBlock)
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(EU3D_CHEM_WAVES_PER_EU)))
     k_chem_queue(Grid g, const DevMech *__restrict__ mp, double dtm, double *__restrict__ CS,
                  double *__restrict__ prim, double *__restrict__ T, double *__restrict__ E, double *__restrict__ C,
                  double *__restrict__ CMS, const int *__restrict__ nchem, const int *__restrict__ queue,
@@ -644,7 +699,7 @@ __global__ void __launch_bounds__(kBlock)
 
 // Uncompacted: every interior cell runs its own remaining sub-steps.
 template <int NS>
-__global__ void __launch_bounds__(kBlock)
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(EU3D_CHEM_WAVES_PER_EU)))
     k_chem_all(Grid g, const DevMech *__restrict__ mp, double dtm, double *__restrict__ CS,
                double *__restrict__ prim, double *__restrict__ T, double *__restrict__ E, double *__restrict__ C,
                double *__restrict__ CMS, const int *__restrict__ nchem, int keep_cms)
@@ -672,10 +727,10 @@ __global__ void k_probe_thermo(const DevMech *__restrict__ mp, const double *Tin
     int it = t / NS, sp = t % NS;
     double T = Tin[it];
     TPow tp = tpow(T);
-    const double *a = coeffs(*mp, T, sp);
-    cp[t] = getcpi(tp, R, a, NS);
-    h[t] = gethi(tp, R, a, NS);
-    s[t] = getsi(tp, R, a, NS);
+    const CoefSel a = coeffs(*mp, T);
+    cp[t] = getcpi(tp, R, a, sp, NS);
+    h[t] = gethi(tp, R, a, sp, NS);
+    s[t] = getsi(tp, R, a, sp, NS);
 }
 
 template <int NS>
@@ -728,10 +783,11 @@ __global__ void k_derive_cp(Grid g, const DevMech *__restrict__ mp, const double
     const DevMech &m = *mp;
     double Tc = T[c];
     TPow tp = tpow(Tc);
+    const CoefSel a = coeffs(m, Tc);
     double scp = 0.0;
 #pragma unroll
     for (int s = 0; s < NS; s++)
-        scp += getcpi(tp, m.ri[s], coeffs(m, Tc, s), NS) * prim[(7 + s) * g.N + c];
+        scp += getcpi(tp, m.ri[s], a, s, NS) * prim[(7 + s) * g.N + c];
     cp_out[c] = scp * 1000;
 }
 
@@ -750,20 +806,24 @@ __global__ void k_derive_cp(Grid g, const DevMech *__restrict__ mp, const double
 template <int NS>
 static void launch_flux(eu3d_ctx *ctx, int mode, double dt)
 {
-    unsigned int nb = blocks_for(ctx->n_int);
     const Grid &g = ctx->g;
+    dim3 grid((g.nk + kTK - 1) / kTK, (g.nj + kTJ - 1) / kTJ, (g.ni + kMarch - 1) / kMarch);
     switch (mode) {
     case 0:
-        k_flux_update<NS, 0><<<nb, kBlock, 0, ctx->stream>>>(g, ctx->prim, ctx->CS, ctx->CS0, ctx->x, ctx->y, ctx->z, dt);
+        k_flux_tile<NS, 0><<<grid, kFluxThreads, 0, ctx->stream>>>(g, ctx->prim, ctx->CS, ctx->CS0, ctx->x, ctx->y,
+                                                                    ctx->z, dt);
         break;
     case 1:
-        k_flux_update<NS, 1><<<nb, kBlock, 0, ctx->stream>>>(g, ctx->prim, ctx->CS, ctx->CS0, ctx->x, ctx->y, ctx->z, dt);
+        k_flux_tile<NS, 1><<<grid, kFluxThreads, 0, ctx->stream>>>(g, ctx->prim, ctx->CS, ctx->CS0, ctx->x, ctx->y,
+                                                                    ctx->z, dt);
         break;
     case 2:
-        k_flux_update<NS, 2><<<nb, kBlock, 0, ctx->stream>>>(g, ctx->prim, ctx->CS, ctx->CS0, ctx->x, ctx->y, ctx->z, dt);
+        k_flux_tile<NS, 2><<<grid, kFluxThreads, 0, ctx->stream>>>(g, ctx->prim, ctx->CS, ctx->CS0, ctx->x, ctx->y,
+                                                                    ctx->z, dt);
         break;
     default:
-        k_flux_update<NS, 3><<<nb, kBlock, 0, ctx->stream>>>(g, ctx->prim, ctx->CS, ctx->CS0, ctx->x, ctx->y, ctx->z, dt);
+        k_flux_tile<NS, 3><<<grid, kFluxThreads, 0, ctx->stream>>>(g, ctx->prim, ctx->CS, ctx->CS0, ctx->x, ctx->y,
+                                                                    ctx->z, dt);
         break;
     }
 }
