@@ -97,6 +97,8 @@ def main():
     ap.add_argument("--ctrl", default=None, help="ctrl file (default: input/ctrl-3dSBI.txt of the package)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--generic-mechanism", action="store_true",
+                    help="run the chemistry on the run-time mechanism tables (EU3D_OPT_GENERIC_MECHANISM)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -109,6 +111,10 @@ def main():
     ctrl = args.ctrl or os.path.join(base, "input", "ctrl-3dSBI.txt")
     s = eu3d.Solver(ctrl, base_dir=base)
     ctx = s.ctx
+    if args.generic_mechanism:
+        eu3d.check(L.eu3d_set_option(ctx, eu3d.OPT_GENERIC_MECHANISM, 1), ctx)
+    gen = eu3d.ctypes.c_int(0)
+    eu3d.check(L.eu3d_get_option(ctx, eu3d.OPT_GENERIC_MECHANISM, eu3d.ctypes.byref(gen)), ctx)
     n_int = s.ni * s.nj * s.nk
     flags = eu3d.CHEM_COMPACT | eu3d.CHEM_KEEP_CMS
     print(f"bench: {s.ni}x{s.nj}x{s.nk} nodes, warmup {args.warmup}, steps {args.steps}", file=sys.stderr,
@@ -169,7 +175,8 @@ def main():
         "config": {"workload": "3D reactive shock-bubble, plain FP64 path (configs[1])",
                    "grid": [s.ni, s.nj, s.nk], "species": s.ns, "reactions": s.nr,
                    "scheme": "SSP-RK3 + MUSCL(minmod) + AUSM+", "chemistry": "adaptive trapezoid, compacted",
-                   "cfl": 0.5, "sim_time_end": info["time"]},
+                   "cfl": 0.5, "sim_time_end": info["time"],
+                   "mechanism_kernels": "run-time tables" if gen.value else "compiled-in H2_JA919"},
         "roofline": roofline,
         "roofline_flux": roof_flux,
         "roofline_chem": roof_chem,

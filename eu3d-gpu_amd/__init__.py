@@ -25,6 +25,8 @@ SCHEME_TVD_RK3 = 1
 CHEM_COMPACT = 0x1
 CHEM_SKIP = 0x2
 CHEM_KEEP_CMS = 0x4
+OPT_GENERIC_MECHANISM = 1
+OPT_GENERATED_AVAILABLE = 2
 
 _vp = ctypes.c_void_p
 _i = ctypes.c_int
@@ -78,6 +80,8 @@ SIGNATURES = {
     "eu3d_run": (_i, [_vp, _i, _d, _d, _d, _i, _i, ctypes.POINTER(_d), ctypes.POINTER(_i)]),
     "eu3d_health": (_i, [_vp, ctypes.POINTER(Health)]),
     "eu3d_synchronize": (_i, [_vp]),
+    "eu3d_set_option": (_i, [_vp, _i, _i]),
+    "eu3d_get_option": (_i, [_vp, _i, ctypes.POINTER(_i)]),
     "eu3d_timing_get": (_i, [_vp, ctypes.POINTER(Timing)]),
     "eu3d_timing_enable": (_i, [_vp, _i]),
     "eu3d_probe_thermo": (_i, [_vp, _vp, _i, _d, _vp, _vp, _vp]),

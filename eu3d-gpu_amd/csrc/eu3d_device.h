@@ -19,6 +19,7 @@ constexpr int kMaxNR = 24;
 // Mechanism + thermo tables, resident in device memory, read with uniform
 // (scalar) loads.  Layouts follow Reaction::ReactionRead (src/Reaction.cpp:156-329).
 struct DevMech {
+    static constexpr bool kGenerated = false;
     int ns, nr;
     double R, Ru, P0;
     double mw[kMaxNS], ri[kMaxNS];
@@ -27,7 +28,16 @@ struct DevMech {
     double nf[kMaxNS * kMaxNR], nb[kMaxNS * kMaxNR], tb[kMaxNS * kMaxNR]; // s*nr + r
     double p0x[kMaxNR]; // pow(P0*1e-6, Xr), evaluated on the host with the C library
     double mxr[kMaxNR]; // -Xr, Xr = sum_s (nb - nf) in species order
+
+    __device__ __forceinline__ const double *thermo_low() const { return c0; }
+    __device__ __forceinline__ const double *thermo_high() const { return c1; }
 };
+
+} // namespace eu3d
+
+#include "mech_h2_ja919.h"
+
+namespace eu3d {
 
 // Grid description passed by value to kernels.
 struct Grid {
@@ -86,43 +96,46 @@ __device__ __forceinline__ TPow tpow(double T)
     return t;
 }
 
-// NASA-9 coefficients of one temperature range.  Both ranges are read with
-// uniform (scalar) loads and selected per lane, so no lane-dependent address
-// is formed for the table.
-struct CoefSel {
-    const double *c0, *c1;
-    bool low; // T < 1000 K (include/ThermoConstants.hpp:3)
+// NASA-9 coefficients of one temperature range: coefficient k of species s
+// at index k*ns + s of the range's table.
+struct CoefPtr {
+    const double *a;
+    __device__ __forceinline__ double operator()(int idx) const { return a[idx]; }
 };
 
-__device__ __forceinline__ CoefSel coeffs(const DevMech &m, double T)
+// Calls f(coefficients) for the temperature range of T (T < 1000 K: low,
+// include/ThermoConstants.hpp:3).  Each lane reads its own range's table
+// with vector loads: uniform (scalar) operands would be hoisted out of the
+// Newton and sub-step loops by the compiler and spilled.
+template <class MT, class F>
+__device__ __forceinline__ void by_range(const MT &m, double T, F &&f)
 {
-    return CoefSel{m.c0, m.c1, T < EU3D_THERMO_SWITCH_T};
+    const bool low = T < EU3D_THERMO_SWITCH_T;
+    const double *p = low ? m.thermo_low() : m.thermo_high();
+    // keep the table loads at their use: hoisted out of the Newton loop
+    // they would hold 162 values in registers for the whole loop
+    asm volatile("" : "+v"(p));
+    f(CoefPtr{p});
 }
 
-__device__ __forceinline__ double cf(const CoefSel &c, int idx)
+// Reaction::GetCpi / GetHi / GetSi (src/Reaction.cpp:1165-1220) of species s.
+template <class C>
+__device__ __forceinline__ double getcpi(const TPow &t, double R, const C &c, int s, int ns)
 {
-    double a = c.c0[idx], b = c.c1[idx];
-    return c.low ? a : b;
+    return R * (c(s) * t.Tm2 + c(ns + s) * t.Tm1 + c(2 * ns + s) + c(3 * ns + s) * t.T + c(4 * ns + s) * t.T2 +
+                c(5 * ns + s) * t.T3 + c(6 * ns + s) * t.T4);
 }
-
-// Reaction::GetCpi / GetHi / GetSi (src/Reaction.cpp:1165-1220) of species s
-// (coefficient k of species s at index k*ns + s).
-__device__ __forceinline__ double getcpi(const TPow &t, double R, const CoefSel &c, int s, int ns)
+template <class C>
+__device__ __forceinline__ double gethi(const TPow &t, double R, const C &c, int s, int ns)
 {
-    return R * (cf(c, s) * t.Tm2 + cf(c, ns + s) * t.Tm1 + cf(c, 2 * ns + s) + cf(c, 3 * ns + s) * t.T +
-                cf(c, 4 * ns + s) * t.T2 + cf(c, 5 * ns + s) * t.T3 + cf(c, 6 * ns + s) * t.T4);
+    return R * (-c(s) * t.Tm1 + c(ns + s) * t.lnT + c(2 * ns + s) * t.T + c(3 * ns + s) * t.T2 / 2 +
+                c(4 * ns + s) * t.T3 / 3 + c(5 * ns + s) * t.T4 / 4 + c(6 * ns + s) * t.T5 / 5 + c(7 * ns + s));
 }
-__device__ __forceinline__ double gethi(const TPow &t, double R, const CoefSel &c, int s, int ns)
+template <class C>
+__device__ __forceinline__ double getsi(const TPow &t, double R, const C &c, int s, int ns)
 {
-    return R * (-cf(c, s) * t.Tm1 + cf(c, ns + s) * t.lnT + cf(c, 2 * ns + s) * t.T + cf(c, 3 * ns + s) * t.T2 / 2 +
-                cf(c, 4 * ns + s) * t.T3 / 3 + cf(c, 5 * ns + s) * t.T4 / 4 + cf(c, 6 * ns + s) * t.T5 / 5 +
-                cf(c, 7 * ns + s));
-}
-__device__ __forceinline__ double getsi(const TPow &t, double R, const CoefSel &c, int s, int ns)
-{
-    return R * (-cf(c, s) * t.Tm2 / 2 - cf(c, ns + s) * t.Tm1 + cf(c, 2 * ns + s) * t.lnT + cf(c, 3 * ns + s) * t.T +
-                cf(c, 4 * ns + s) * t.T2 / 2 + cf(c, 5 * ns + s) * t.T3 / 3 + cf(c, 6 * ns + s) * t.T4 / 4 +
-                cf(c, 8 * ns + s));
+    return R * (-c(s) * t.Tm2 / 2 - c(ns + s) * t.Tm1 + c(2 * ns + s) * t.lnT + c(3 * ns + s) * t.T +
+                c(4 * ns + s) * t.T2 / 2 + c(5 * ns + s) * t.T3 / 3 + c(6 * ns + s) * t.T4 / 4 + c(8 * ns + s));
 }
 
 // ---------------------------------------------------------------------------
@@ -130,8 +143,8 @@ __device__ __forceinline__ double getsi(const TPow &t, double R, const CoefSel &
 // sum Y h(T) - E*1e-3 - (sum Y R) T with derivative sum Y cp - sum Y R; at
 // most 10 iterations, stop when |dT| < 1e-6.  E is the previous internal energy.
 // ---------------------------------------------------------------------------
-template <int NS>
-__device__ __forceinline__ double get_temp(const DevMech &m, double T, double E, const double (&Y)[NS])
+template <int NS, class MT>
+__device__ __forceinline__ double get_temp(const MT &m, double T, double E, const double (&Y)[NS])
 {
     double T0 = T, T_temp = T;
     int count = 0;
@@ -142,15 +155,16 @@ __device__ __forceinline__ double get_temp(const DevMech &m, double T, double E,
     while (count < 10) {
         T0 = T_temp;
         TPow t = tpow(T0);
-        const CoefSel a = coeffs(m, T0);
         double sh = 0.0, scp = 0.0;
+        by_range(m, T0, [&](const auto &a) {
 #pragma unroll
-        for (int s = 0; s < NS; s++) {
-            double cp = getcpi(t, m.ri[s], a, s, NS);
-            double h = gethi(t, m.ri[s], a, s, NS);
-            sh += Y[s] * h;
-            scp += Y[s] * cp;
-        }
+            for (int s = 0; s < NS; s++) {
+                double cp = getcpi(t, m.ri[s], a, s, NS);
+                double h = gethi(t, m.ri[s], a, s, NS);
+                sh += Y[s] * h;
+                scp += Y[s] * cp;
+            }
+        });
         double temp1 = (sh - E * 1e-3) - sYR * T0;
         double temp2 = scp - sYR;
         T_temp = T0 - temp1 / temp2;
@@ -170,8 +184,8 @@ struct CellState {
 
 // Flowfield::Update_after_CS for one point: cs[NS+4] conserved, T and E the
 // previous temperature and internal energy.
-template <int NS>
-__device__ __forceinline__ void recover(const DevMech &m, const double (&cs)[NS + 4], double Told, double Eold,
+template <int NS, class MT>
+__device__ __forceinline__ void recover(const MT &m, const double (&cs)[NS + 4], double Told, double Eold,
                                         CellState<NS> &o)
 {
     const double R = m.R;
@@ -194,11 +208,12 @@ __device__ __forceinline__ void recover(const DevMech &m, const double (&cs)[NS 
     double P = T * D * Rgas;
     double H = P / D + E;
     TPow t = tpow(T);
-    const CoefSel a = coeffs(m, T);
     double scp = 0.0;
+    by_range(m, T, [&](const auto &a) {
 #pragma unroll
-    for (int s = 0; s < NS; s++)
-        scp += getcpi(t, m.ri[s], a, s, NS) * o.Y[s];
+        for (int s = 0; s < NS; s++)
+            scp += getcpi(t, m.ri[s], a, s, NS) * o.Y[s];
+    });
     double Cp = scp * 1000;
     double G = Cp / (Cp - R * sYM * 1000);
     double C = sqrt(G * P / D);
@@ -360,22 +375,66 @@ struct Rates {
     double P[NS]; // trapezoid loss term, before the division by Mc
 };
 
-// Mc in mol/cm^3.  If want_pq, Q and P sums are formed as well.
-template <int NS, bool want_pq>
-__device__ __forceinline__ void rates(const DevMech &m, double T, const double (&Mc)[NS], Rates<NS> &o)
+// One reaction's contribution: rate constants, rates of progress and the
+// per-species sums (src/Reaction.cpp:609-631, 655-662, 781-793).
+template <int NS, bool want_pq, class MT>
+__device__ __forceinline__ void reaction_term(const MT &m, int r, double T, const double (&Gi)[NS],
+                                              const double (&Mc)[NS], Rates<NS> &o)
 {
     const int nr = m.nr;
     const double R = m.R, Ru = m.Ru;
+    double KF = m.af[r] * pow_small(T, m.bf[r]);
+    if (m.eaf[r] != 0.0)
+        KF = KF * exp(-m.eaf[r] / (Ru * T));
+    double delta_g = 0.0, RRF = 1.0, RRB = 1.0, RTB = 0.0;
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+        const double nb = m.nb[s * nr + r], nf = m.nf[s * nr + r], tb = m.tb[s * nr + r];
+        if (nb != nf)
+            delta_g += (nb - nf) * Gi[s];
+        if (nf != 0.0)
+            RRF *= pow_small(Mc[s], nf);
+        if (nb != 0.0)
+            RRB *= pow_small(Mc[s], nb);
+        if (tb != 0.0)
+            RTB += Mc[s] * tb;
+    }
+    double Kp = exp(-delta_g / (R * T)) * m.p0x[r];
+    double Kc = Kp * pow_small(R * T, m.mxr[r]);
+    double KB = KF / Kc;
+    if (RTB == 0.0)
+        RTB = 1.0;
+    double RR = KF * RRF - KB * RRB;
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+        const double nb = m.nb[s * nr + r], nf = m.nf[s * nr + r];
+        if (nb != nf)
+            o.W[s] += (nb - nf) * RTB * RR;
+        if (want_pq && (nb != 0.0 || nf != 0.0)) {
+            o.Q[s] += (nb * KF * RRF + nf * KB * RRB) * RTB;
+            o.P[s] += (nb * KB * RRB + nf * KF * RRF) * RTB;
+        }
+    }
+}
+
+// Mc in mol/cm^3.  If want_pq, Q and P sums are formed as well.  With a
+// generated mechanism type the reaction loop is unrolled and every table
+// entry is a compile-time constant.
+template <int NS, bool want_pq, class MT>
+__device__ __forceinline__ void rates(const MT &m, double T, const double (&Mc)[NS], Rates<NS> &o)
+{
+    const double R = m.R;
     double Gi[NS];
     {
         TPow t = tpow(T);
-        const CoefSel a = coeffs(m, T);
+        by_range(m, T, [&](const auto &a) {
 #pragma unroll
-        for (int s = 0; s < NS; s++) {
-            double Hi = gethi(t, R, a, s, NS);
-            double Si = getsi(t, R, a, s, NS);
-            Gi[s] = Hi - Si * T;
-        }
+            for (int s = 0; s < NS; s++) {
+                double Hi = gethi(t, R, a, s, NS);
+                double Si = getsi(t, R, a, s, NS);
+                Gi[s] = Hi - Si * T;
+            }
+        });
     }
 #pragma unroll
     for (int s = 0; s < NS; s++) {
@@ -385,40 +444,14 @@ __device__ __forceinline__ void rates(const DevMech &m, double T, const double (
             o.P[s] = 0.0;
         }
     }
+    if constexpr (MT::kGenerated) {
+#pragma unroll
+        for (int r = 0; r < MT::nr; r++)
+            reaction_term<NS, want_pq>(m, r, T, Gi, Mc, o);
+    } else {
 #pragma unroll 1
-    for (int r = 0; r < nr; r++) {
-        double KF = m.af[r] * pow_small(T, m.bf[r]);
-        if (m.eaf[r] != 0.0)
-            KF = KF * exp(-m.eaf[r] / (Ru * T));
-        double delta_g = 0.0, RRF = 1.0, RRB = 1.0, RTB = 0.0;
-#pragma unroll
-        for (int s = 0; s < NS; s++) {
-            const double nb = m.nb[s * nr + r], nf = m.nf[s * nr + r], tb = m.tb[s * nr + r];
-            if (nb != nf)
-                delta_g += (nb - nf) * Gi[s];
-            if (nf != 0.0)
-                RRF *= pow_small(Mc[s], nf);
-            if (nb != 0.0)
-                RRB *= pow_small(Mc[s], nb);
-            if (tb != 0.0)
-                RTB += Mc[s] * tb;
-        }
-        double Kp = exp(-delta_g / (R * T)) * m.p0x[r];
-        double Kc = Kp * pow_small(R * T, m.mxr[r]);
-        double KB = KF / Kc;
-        if (RTB == 0.0)
-            RTB = 1.0;
-        double RR = KF * RRF - KB * RRB;
-#pragma unroll
-        for (int s = 0; s < NS; s++) {
-            const double nb = m.nb[s * nr + r], nf = m.nf[s * nr + r];
-            if (nb != nf)
-                o.W[s] += (nb - nf) * RTB * RR;
-            if (want_pq && (nb != 0.0 || nf != 0.0)) {
-                o.Q[s] += (nb * KF * RRF + nf * KB * RRB) * RTB;
-                o.P[s] += (nb * KB * RRB + nf * KF * RRF) * RTB;
-            }
-        }
+        for (int r = 0; r < m.nr; r++)
+            reaction_term<NS, want_pq>(m, r, T, Gi, Mc, o);
     }
 #pragma unroll
     for (int s = 0; s < NS; s++)

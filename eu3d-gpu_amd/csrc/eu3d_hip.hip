@@ -56,6 +56,8 @@ struct eu3d_ctx {
     int *nchem, *queue;
     ChemCounters *cnt;
     bool has_state, cms_valid;
+    bool generated; // tables bit-identical to MechH2JA919: use the compile-time kernels
+    bool generated_available;
     std::string err;
     bool timing;
     std::vector<TimedKind> pending;
@@ -153,6 +155,18 @@ __device__ __forceinline__ bool interior_cell(const Grid &g, long long t, int &i
 
 static unsigned int blocks_for(long long n, int b = kBlock) { return (unsigned int)((n + b - 1) / b); }
 
+// The mechanism seen by a kernel: the run-time tables, or a generated type
+// whose reaction tables are static constexpr members (it keeps the run-time
+// pointer for the thermo tables).
+template <class MT>
+__device__ __forceinline__ decltype(auto) mech_ref(const DevMech *p)
+{
+    if constexpr (MT::kGenerated)
+        return MT{p};
+    else
+        return static_cast<const DevMech &>(*p);
+}
+
 // ---------------------------------------------------------------------------
 // ghost fill (src/Flowfield.cpp:593-807)
 // ---------------------------------------------------------------------------
@@ -192,7 +206,7 @@ __global__ void __launch_bounds__(kBlock) k_ghost_x(Grid g, int nprim, double *p
 }
 
 // y faces: front symmetry (v -> -v, thermo recomputed), back copy; all i, k interior
-template <int NS>
+template <int NS, class MT>
 __global__ void __launch_bounds__(kBlock)
     k_ghost_y(Grid g, const DevMech *__restrict__ mp, double *prim, double *T, double *E)
 {
@@ -213,7 +227,7 @@ __global__ void __launch_bounds__(kBlock)
         copy_point(g, 7 + NS, prim, T, E, gidx(g, i, j, k), gidx(g, i, js, k), -1);
         return;
     }
-    const DevMech &m = *mp;
+    const auto &m = mech_ref<MT>(mp);
     int j = jj, js = 2 * g.bc - 1 - j;
     long long d = gidx(g, i, j, k), s = gidx(g, i, js, k);
     double Y[NS];
@@ -230,20 +244,21 @@ __global__ void __launch_bounds__(kBlock)
         prim[(7 + q) * N + d] = Y[q];
     }
     TPow tp = tpow(Tt);
-    const CoefSel a = coeffs(m, Tt);
     double sR = 0.0, sCp = 0.0, sH = 0.0, sE = 0.0;
 #pragma unroll
     for (int q = 0; q < NS; q++)
         sR += Y[q] * m.ri[q];
+    by_range(m, Tt, [&](const auto &a) {
 #pragma unroll
-    for (int q = 0; q < NS; q++) {
-        double cp = getcpi(tp, m.ri[q], a, q, NS);
-        double h = gethi(tp, m.ri[q], a, q, NS);
-        double e = h - m.ri[q] * Tt;
-        sCp += cp * Y[q];
-        sH += Y[q] * h;
-        sE += Y[q] * e;
-    }
+        for (int q = 0; q < NS; q++) {
+            double cp = getcpi(tp, m.ri[q], a, q, NS);
+            double h = gethi(tp, m.ri[q], a, q, NS);
+            double e = h - m.ri[q] * Tt;
+            sCp += cp * Y[q];
+            sH += Y[q] * h;
+            sE += Y[q] * e;
+        }
+    });
     double Rgas = sR * 1000;
     double D = P / (Tt * Rgas);
     double Cp = sCp * 1000;
@@ -413,7 +428,7 @@ __device__ __forceinline__ void store_state(long long N, long long c, const Cell
     C[c] = st.C;
 }
 
-template <int NS>
+template <int NS, class MT>
 __global__ void __launch_bounds__(kBlock)
     k_recover(Grid g, const DevMech *__restrict__ mp, const double *__restrict__ CS, double *__restrict__ prim,
               double *__restrict__ T, double *__restrict__ E, double *__restrict__ C)
@@ -428,7 +443,7 @@ __global__ void __launch_bounds__(kBlock)
     for (int q = 0; q < NS + 4; q++)
         cs[q] = CS[q * N + c];
     CellState<NS> st;
-    recover<NS>(*mp, cs, T[c], E[c], st);
+    recover<NS>(mech_ref<MT>(mp), cs, T[c], E[c], st);
     store_state<NS>(N, c, st, prim, T, E, C);
 }
 
@@ -514,8 +529,8 @@ __device__ __forceinline__ void chem_load(const Grid &g, long long c, const doub
 // One sub-step: trapezoid update (src/Reaction.cpp:781-805) with rates R
 // already evaluated at the current state, then Explicit(step) +
 // Update_after_CS(step) (src/Flowfield.cpp:1522-1536, 1620-1676).
-template <int NS>
-__device__ __forceinline__ void chem_substep(const DevMech &m, ChemCell<NS> &x, const Rates<NS> &R,
+template <int NS, class MT>
+__device__ __forceinline__ void chem_substep(const MT &m, ChemCell<NS> &x, const Rates<NS> &R,
                                              const double (&Mc)[NS], double dt, double (&cms)[NS],
                                              double (&cs)[NS + 4], CellState<NS> &st)
 {
@@ -549,8 +564,8 @@ __device__ __forceinline__ void chem_substep(const DevMech &m, ChemCell<NS> &x, 
     x.W = st.W;
 }
 
-template <int NS>
-__device__ __forceinline__ void chem_mc(const DevMech &m, const ChemCell<NS> &x, double (&Mc)[NS])
+template <int NS, class MT>
+__device__ __forceinline__ void chem_mc(const MT &m, const ChemCell<NS> &x, double (&Mc)[NS])
 {
     // Mc = Di/Mw*1000 (Update_after_CS) then *1e-6 (Reaction::Trapezoid)
 #pragma unroll
@@ -584,18 +599,59 @@ __device__ __forceinline__ unsigned long long lanemask_lt()
     return lane == 0 ? 0ull : (~0ull >> (64 - lane));
 }
 
+// One sub-step of cell c from the state in memory, written back to memory.
+// FIRST: the rates also give Nchem (TrapezoidPrediction, src/Reaction.cpp:
+// 865-942), and the sub-step uses them (the reference evaluates the same
+// rates twice on identical inputs).  Returns Nchem (0: invalid, no update).
+// The rest of the cell state is read after the rates so that it is not held
+// in registers through them.
+template <int NS, bool FIRST, class MT>
+__device__ __forceinline__ int chem_step_cell(const Grid &g, const MT &m, long long c, double dtm, int n,
+                                              double *CS, double *prim, double *T, double *E, double *C,
+                                              double *CMS, int keep_cms)
+{
+    const long long N = g.N;
+    const double Tc = T[c];
+    double Mc[NS];
+#pragma unroll
+    for (int s = 0; s < NS; s++)
+        Mc[s] = CS[s * N + c] / m.mw[s] * 1000 * 1e-6; // Update_after_CS, then Reaction::Trapezoid
+    Rates<NS> R;
+    rates<NS, true>(m, Tc, Mc, R);
+    long long cl = c;
+    asm volatile("" : "+v"(cl) : "v"(R.W[0]));
+    ChemCell<NS> x;
+    chem_load<NS>(g, cl, CS, prim, T, E, x);
+    if (FIRST) {
+        double Dcgs[NS], Y[NS];
+#pragma unroll
+        for (int s = 0; s < NS; s++) {
+            Dcgs[s] = x.Di[s] * 1e-3;
+            Y[s] = prim[(7 + s) * N + cl];
+        }
+        n = predict_nchem<NS>(R.W, Dcgs, Y, dtm);
+        if (n < 1)
+            return n;
+    }
+    double cms[NS], cs[NS + 4];
+    CellState<NS> st;
+    chem_substep<NS>(m, x, R, Mc, dtm / n, cms, cs, st);
+    chem_store<NS>(N, cl, cs, st, cms, keep_cms, CS, CMS, prim, T, E, C);
+    return n;
+}
+
 // First chemistry pass over every interior cell: rates at the post-advection
 // state, Nchem (TrapezoidPrediction), and sub-step 0 with the same rates (the
 // reference evaluates them twice on identical inputs).  Cells with Nchem > 1
 // are appended to the stiff queue.
-template <int NS>
+template <int NS, class MT>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(EU3D_CHEM_WAVES_PER_EU)))
     k_chem_first(Grid g, const DevMech *__restrict__ mp, double dtm, double *__restrict__ CS,
                  double *__restrict__ prim, double *__restrict__ T, double *__restrict__ E, double *__restrict__ C,
                  double *__restrict__ CMS, int *__restrict__ nchem, int *__restrict__ queue, ChemCounters *cnt,
                  int compact, int keep_cms)
 {
-    const DevMech &m = *mp;
+    const auto &m = mech_ref<MT>(mp);
     long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
     int i = 0, j = 0, k = 0;
     bool active = interior_cell(g, t, i, j, k);
@@ -603,28 +659,12 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(EU3
     int n = 0;
     bool stiff = false;
     if (active) {
-        ChemCell<NS> x;
-        chem_load<NS>(g, c, CS, prim, T, E, x);
-        double Mc[NS], Dcgs[NS], Y[NS];
-        chem_mc<NS>(m, x, Mc);
-#pragma unroll
-        for (int s = 0; s < NS; s++) {
-            Dcgs[s] = x.Di[s] * 1e-3;
-            Y[s] = prim[(7 + s) * g.N + c];
-        }
-        Rates<NS> R;
-        rates<NS, true>(m, x.T, Mc, R);
-        n = predict_nchem<NS>(R.W, Dcgs, Y, dtm);
+        n = chem_step_cell<NS, true>(g, m, c, dtm, 0, CS, prim, T, E, C, CMS, keep_cms);
         nchem[c] = n;
-        if (n >= 1) {
-            double cms[NS], cs[NS + 4];
-            CellState<NS> st;
-            chem_substep<NS>(m, x, R, Mc, dtm / n, cms, cs, st);
-            chem_store<NS>(g.N, c, cs, st, cms, keep_cms, CS, CMS, prim, T, E, C);
+        if (n >= 1)
             stiff = n > 1;
-        } else {
+        else
             atomicAdd(&cnt->invalid, 1ull);
-        }
     }
     // wave-aggregated append of stiff cells
     unsigned long long mask = __ballot(stiff);
@@ -652,28 +692,17 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(EU3
 }
 
 // Remaining sub-steps 1..Nchem-1 of one cell.
-template <int NS>
-__device__ __forceinline__ void chem_rest(const Grid &g, const DevMech &m, long long c, int n, double dtm,
+template <int NS, class MT>
+__device__ __forceinline__ void chem_rest(const Grid &g, const MT &m, long long c, int n, double dtm,
                                           double *CS, double *prim, double *T, double *E, double *C, double *CMS,
                                           int keep_cms)
 {
-    ChemCell<NS> x;
-    chem_load<NS>(g, c, CS, prim, T, E, x);
-    double cms[NS], cs[NS + 4];
-    CellState<NS> st;
-    double dt = dtm / n;
-    for (int step = 1; step < n; step++) {
-        double Mc[NS];
-        chem_mc<NS>(m, x, Mc);
-        Rates<NS> R;
-        rates<NS, true>(m, x.T, Mc, R);
-        chem_substep<NS>(m, x, R, Mc, dt, cms, cs, st);
-    }
-    chem_store<NS>(g.N, c, cs, st, cms, keep_cms, CS, CMS, prim, T, E, C);
+    for (int step = 1; step < n; step++)
+        chem_step_cell<NS, false>(g, m, c, dtm, n, CS, prim, T, E, C, CMS, keep_cms);
 }
 
 // Compacted: waves fetch 64 queued cells at a time until the queue is drained.
-template <int NS>
+template <int NS, class MT>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(EU3D_CHEM_WAVES_PER_EU)))
     k_chem_queue(Grid g, const DevMech *__restrict__ mp, double dtm, double *__restrict__ CS,
                  double *__restrict__ prim, double *__restrict__ T, double *__restrict__ E, double *__restrict__ C,
@@ -692,13 +721,13 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(EU3
         unsigned int q = base + lane;
         if (q < total) {
             long long c = queue[q];
-            chem_rest<NS>(g, *mp, c, nchem[c], dtm, CS, prim, T, E, C, CMS, keep_cms);
+            chem_rest<NS>(g, mech_ref<MT>(mp), c, nchem[c], dtm, CS, prim, T, E, C, CMS, keep_cms);
         }
     }
 }
 
 // Uncompacted: every interior cell runs its own remaining sub-steps.
-template <int NS>
+template <int NS, class MT>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(EU3D_CHEM_WAVES_PER_EU)))
     k_chem_all(Grid g, const DevMech *__restrict__ mp, double dtm, double *__restrict__ CS,
                double *__restrict__ prim, double *__restrict__ T, double *__restrict__ E, double *__restrict__ C,
@@ -711,13 +740,13 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(EU3
     long long c = gidx(g, i, j, k);
     int n = nchem[c];
     if (n > 1)
-        chem_rest<NS>(g, *mp, c, n, dtm, CS, prim, T, E, C, CMS, keep_cms);
+        chem_rest<NS>(g, mech_ref<MT>(mp), c, n, dtm, CS, prim, T, E, C, CMS, keep_cms);
 }
 
 // ---------------------------------------------------------------------------
 // probes
 // ---------------------------------------------------------------------------
-template <int NS>
+template <int NS, class MT>
 __global__ void k_probe_thermo(const DevMech *__restrict__ mp, const double *Tin, int n, double R, double *cp,
                                double *h, double *s)
 {
@@ -727,7 +756,8 @@ __global__ void k_probe_thermo(const DevMech *__restrict__ mp, const double *Tin
     int it = t / NS, sp = t % NS;
     double T = Tin[it];
     TPow tp = tpow(T);
-    const CoefSel a = coeffs(*mp, T);
+    const auto &m = mech_ref<MT>(mp);
+    const CoefPtr a{T < EU3D_THERMO_SWITCH_T ? m.c0 : m.c1};
     cp[t] = getcpi(tp, R, a, sp, NS);
     h[t] = gethi(tp, R, a, sp, NS);
     s[t] = getsi(tp, R, a, sp, NS);
@@ -754,7 +784,7 @@ __global__ void k_probe_flux(Grid g, int dir, const double *__restrict__ prim, d
         out[q * g.N + a] = F[q];
 }
 
-template <int NS>
+template <int NS, class MT>
 __global__ void k_probe_rates(const DevMech *__restrict__ mp, const double *Tin, const double *Mcin, int n,
                               double *wi)
 {
@@ -766,28 +796,29 @@ __global__ void k_probe_rates(const DevMech *__restrict__ mp, const double *Tin,
     for (int s = 0; s < NS; s++)
         Mc[s] = Mcin[t * NS + s];
     Rates<NS> R;
-    rates<NS, false>(*mp, Tin[t], Mc, R);
+    rates<NS, false>(mech_ref<MT>(mp), Tin[t], Mc, R);
 #pragma unroll
     for (int s = 0; s < NS; s++)
         wi[t * NS + s] = R.W[s];
 }
 
 // Cp of every point for downloads (Update_after_CS, src/Flowfield.cpp:1609-1611)
-template <int NS>
+template <int NS, class MT>
 __global__ void k_derive_cp(Grid g, const DevMech *__restrict__ mp, const double *__restrict__ prim,
                             const double *__restrict__ T, double *__restrict__ cp_out)
 {
     long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x;
     if (c >= g.N)
         return;
-    const DevMech &m = *mp;
+    const auto &m = mech_ref<MT>(mp);
     double Tc = T[c];
     TPow tp = tpow(Tc);
-    const CoefSel a = coeffs(m, Tc);
     double scp = 0.0;
+    by_range(m, Tc, [&](const auto &a) {
 #pragma unroll
-    for (int s = 0; s < NS; s++)
-        scp += getcpi(tp, m.ri[s], a, s, NS) * prim[(7 + s) * g.N + c];
+        for (int s = 0; s < NS; s++)
+            scp += getcpi(tp, m.ri[s], a, s, NS) * prim[(7 + s) * g.N + c];
+    });
     cp_out[c] = scp * 1000;
 }
 
@@ -797,13 +828,16 @@ __global__ void k_derive_cp(Grid g, const DevMech *__restrict__ mp, const double
 #define EU3D_DISPATCH_NS(ns, F, ...)                                                              \
     switch (ns) {                                                                                 \
     case 9:                                                                                       \
-        F<9>(__VA_ARGS__);                                                                        \
+        if (ctx->generated)                                                                       \
+            F<9, MechH2JA919>(__VA_ARGS__);                                                       \
+        else                                                                                      \
+            F<9, DevMech>(__VA_ARGS__);                                                           \
         break;                                                                                    \
     default:                                                                                      \
         FAIL(EU3D_ERR_ARG, "species count not compiled in (supported: 9)");                       \
     }
 
-template <int NS>
+template <int NS, class MT>
 static void launch_flux(eu3d_ctx *ctx, int mode, double dt)
 {
     const Grid &g = ctx->g;
@@ -828,70 +862,112 @@ static void launch_flux(eu3d_ctx *ctx, int mode, double dt)
     }
 }
 
-template <int NS>
+template <int NS, class MT>
 static void launch_recover(eu3d_ctx *ctx)
 {
-    k_recover<NS><<<blocks_for(ctx->n_int), kBlock, 0, ctx->stream>>>(ctx->g, ctx->mech, ctx->CS, ctx->prim, ctx->T,
+    k_recover<NS, MT><<<blocks_for(ctx->n_int), kBlock, 0, ctx->stream>>>(ctx->g, ctx->mech, ctx->CS, ctx->prim, ctx->T,
                                                                        ctx->E, ctx->C);
 }
 
-template <int NS>
+template <int NS, class MT>
 static void launch_ghost_y(eu3d_ctx *ctx)
 {
     long long n = 2LL * ctx->g.NI * ctx->g.bc * ctx->g.nk;
-    k_ghost_y<NS><<<blocks_for(n), kBlock, 0, ctx->stream>>>(ctx->g, ctx->mech, ctx->prim, ctx->T, ctx->E);
+    k_ghost_y<NS, MT><<<blocks_for(n), kBlock, 0, ctx->stream>>>(ctx->g, ctx->mech, ctx->prim, ctx->T, ctx->E);
 }
 
-template <int NS>
+template <int NS, class MT>
 static void launch_chem(eu3d_ctx *ctx, double dt, int flags)
 {
     const Grid &g = ctx->g;
     int compact = (flags & EU3D_CHEM_COMPACT) ? 1 : 0;
     int keep = (flags & EU3D_CHEM_KEEP_CMS) ? 1 : 0;
     k_chem_reset<<<1, 1, 0, ctx->stream>>>(ctx->cnt);
-    k_chem_first<NS><<<blocks_for(ctx->n_int), kBlock, 0, ctx->stream>>>(
+    k_chem_first<NS, MT><<<blocks_for(ctx->n_int), kBlock, 0, ctx->stream>>>(
         g, ctx->mech, dt, ctx->CS, ctx->prim, ctx->T, ctx->E, ctx->C, ctx->CMS, ctx->nchem, ctx->queue, ctx->cnt,
         compact, keep);
     if (compact) {
-        k_chem_queue<NS><<<ctx->ncu * 4, kBlock, 0, ctx->stream>>>(g, ctx->mech, dt, ctx->CS, ctx->prim, ctx->T, ctx->E,
+        k_chem_queue<NS, MT><<<ctx->ncu * 4, kBlock, 0, ctx->stream>>>(g, ctx->mech, dt, ctx->CS, ctx->prim, ctx->T, ctx->E,
                                                               ctx->C, ctx->CMS, ctx->nchem, ctx->queue, ctx->cnt,
                                                               keep);
     } else {
-        k_chem_all<NS><<<blocks_for(ctx->n_int), kBlock, 0, ctx->stream>>>(g, ctx->mech, dt, ctx->CS, ctx->prim,
+        k_chem_all<NS, MT><<<blocks_for(ctx->n_int), kBlock, 0, ctx->stream>>>(g, ctx->mech, dt, ctx->CS, ctx->prim,
                                                                            ctx->T, ctx->E, ctx->C, ctx->CMS,
                                                                            ctx->nchem, keep);
     }
 }
 
-template <int NS>
+template <int NS, class MT>
 static void launch_probe_thermo(eu3d_ctx *ctx, const double *T, int n, double R, double *cp, double *h, double *s)
 {
-    k_probe_thermo<NS><<<blocks_for((long long)n * NS), kBlock, 0, ctx->stream>>>(ctx->mech, T, n, R, cp, h, s);
+    k_probe_thermo<NS, MT><<<blocks_for((long long)n * NS), kBlock, 0, ctx->stream>>>(ctx->mech, T, n, R, cp, h, s);
 }
 
-template <int NS>
+template <int NS, class MT>
 static void launch_probe_flux(eu3d_ctx *ctx, int dir, double *out)
 {
     long long n = (long long)(ctx->g.ni + 1) * (ctx->g.nj + 1) * (ctx->g.nk + 1);
     k_probe_flux<NS><<<blocks_for(n), kBlock, 0, ctx->stream>>>(ctx->g, dir, ctx->prim, out);
 }
 
-template <int NS>
+template <int NS, class MT>
 static void launch_probe_rates(eu3d_ctx *ctx, const double *T, const double *Mc, int n, double *wi)
 {
-    k_probe_rates<NS><<<blocks_for(n), kBlock, 0, ctx->stream>>>(ctx->mech, T, Mc, n, wi);
+    k_probe_rates<NS, MT><<<blocks_for(n), kBlock, 0, ctx->stream>>>(ctx->mech, T, Mc, n, wi);
 }
 
-template <int NS>
+template <int NS, class MT>
 static void launch_derive_cp(eu3d_ctx *ctx, double *out)
 {
-    k_derive_cp<NS><<<blocks_for(ctx->g.N), kBlock, 0, ctx->stream>>>(ctx->g, ctx->mech, ctx->prim, ctx->T, out);
+    k_derive_cp<NS, MT><<<blocks_for(ctx->g.N), kBlock, 0, ctx->stream>>>(ctx->g, ctx->mech, ctx->prim, ctx->T, out);
 }
+
+// True when the run-time tables equal the generated ones bit for bit.
+template <class G>
+static bool tables_equal(const DevMech &m)
+{
+    auto same = [](const double *a, const double *b, int n) { return std::memcmp(a, b, sizeof(double) * n) == 0; };
+    double R = G::R, Ru = G::Ru, P0 = G::P0;
+    return m.ns == G::ns && m.nr == G::nr && same(&m.R, &R, 1) && same(&m.Ru, &Ru, 1) && same(&m.P0, &P0, 1) &&
+           same(m.mw, G::mw, kMaxNS) && same(m.ri, G::ri, kMaxNS) && same(m.c0, G::c0, 9 * kMaxNS) &&
+           same(m.c1, G::c1, 9 * kMaxNS) && same(m.af, G::af, kMaxNR) && same(m.bf, G::bf, kMaxNR) &&
+           same(m.eaf, G::eaf, kMaxNR) && same(m.nf, G::nf, kMaxNS * kMaxNR) &&
+           same(m.nb, G::nb, kMaxNS * kMaxNR) && same(m.tb, G::tb, kMaxNS * kMaxNR) &&
+           same(m.p0x, G::p0x, kMaxNR) && same(m.mxr, G::mxr, kMaxNR);
+}
+
+static bool mech_matches_generated(const DevMech &m) { return tables_equal<MechH2JA919>(m); }
 
 // ---------------------------------------------------------------------------
 // C ABI
 // ---------------------------------------------------------------------------
 extern "C" {
+
+int eu3d_set_option(eu3d_ctx *ctx, int option, int value)
+{
+    if (!ctx)
+        return EU3D_ERR_ARG;
+    if (option == EU3D_OPT_GENERIC_MECHANISM) {
+        ctx->generated = ctx->generated_available && value == 0;
+        return EU3D_OK;
+    }
+    FAIL(EU3D_ERR_ARG, "unknown option");
+}
+
+int eu3d_get_option(eu3d_ctx *ctx, int option, int *value)
+{
+    if (!ctx || !value)
+        return EU3D_ERR_ARG;
+    if (option == EU3D_OPT_GENERIC_MECHANISM) {
+        *value = ctx->generated ? 0 : 1;
+        return EU3D_OK;
+    }
+    if (option == EU3D_OPT_GENERATED_AVAILABLE) {
+        *value = ctx->generated_available ? 1 : 0;
+        return EU3D_OK;
+    }
+    FAIL(EU3D_ERR_ARG, "unknown option");
+}
 
 const char *eu3d_last_error(const eu3d_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
 
@@ -985,6 +1061,8 @@ int eu3d_create(const eu3d_mesh *mesh, const eu3d_mechanism *mc, eu3d_ctx **out)
         CK(hipGetDevice(&dev));
         CK(hipDeviceGetAttribute(&ctx->ncu, hipDeviceAttributeMultiprocessorCount, dev));
     }
+    ctx->generated = mech_matches_generated(m);
+    ctx->generated_available = ctx->generated;
     CK(hipStreamCreate(&ctx->stream));
     CK(hipMalloc(&ctx->mech, sizeof(DevMech)));
     CK(hipMemcpy(ctx->mech, &m, sizeof(DevMech), hipMemcpyHostToDevice));

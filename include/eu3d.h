@@ -142,6 +142,17 @@ int eu3d_run(eu3d_ctx *ctx, int scheme, double cfl, double dt_fixed, double fina
  * invalid Nchem or non-finite cells were found. */
 int eu3d_health(eu3d_ctx *ctx, eu3d_health_info *out);
 
+/* Options.  EU3D_OPT_GENERIC_MECHANISM (read/write): 1 runs the chemistry and
+ * thermo kernels on the run-time mechanism tables even when the tables match
+ * a mechanism compiled into the library (the compiled form is chosen by
+ * default; both give the same bits).  EU3D_OPT_GENERATED_AVAILABLE (read
+ * only): 1 if the tables match a compiled-in mechanism.  No reference
+ * counterpart (the reference has one code path). */
+#define EU3D_OPT_GENERIC_MECHANISM 1
+#define EU3D_OPT_GENERATED_AVAILABLE 2
+int eu3d_set_option(eu3d_ctx *ctx, int option, int value);
+int eu3d_get_option(eu3d_ctx *ctx, int option, int *value);
+
 int eu3d_synchronize(eu3d_ctx *ctx);
 int eu3d_timing_get(eu3d_ctx *ctx, eu3d_timing *out);
 int eu3d_timing_enable(eu3d_ctx *ctx, int on);

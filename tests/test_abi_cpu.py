@@ -94,3 +94,12 @@ def test_dump_format_and_comparer(tmp_path):
     b = str(tmp_path / "b.bin")
     open(b, "wb").write(bytes(raw[:len(raw) - data.size * 8]) + body.tobytes())
     assert compare_states.compare(a, [b], out=open(os.devnull, "w")) == 1
+
+
+def test_generated_mechanism_header_is_current():
+    """csrc/mech_h2_ja919.h is what gen_mech.py makes from the bundled input files."""
+    import importlib.util
+    spec = importlib.util.spec_from_file_location("gen_mech", os.path.join(REPO, "eu3d-gpu_amd", "gen_mech.py"))
+    gm = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(gm)
+    assert open(gm.OUT).read() == gm.generate()
