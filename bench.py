@@ -97,9 +97,22 @@ def main():
     ap.add_argument("--ctrl", default=None, help="ctrl file (default: input/ctrl-3dSBI.txt of the package)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only copy MIB MiB device-to-device once (torch) and exit: the known-size kernel that "
+                         "scales the FETCH_SIZE/WRITE_SIZE counters")
     ap.add_argument("--generic-mechanism", action="store_true",
                     help="run the chemistry on the run-time mechanism tables (EU3D_OPT_GENERIC_MECHANISM)")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        import torch
+        n = args.calibrate_copy * (1 << 20) // 8
+        a = torch.ones(n, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibration_copy_bytes": n * 8}))
+        return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(multi-GPU decomposition is out of scope)", file=sys.stderr)
