@@ -1,8 +1,9 @@
 """MI355X (gfx950) reactive-Euler step: Python access to the C ABI.
 
 libeu3d.so (built by build.py) holds the HIP kernels and the C ABI declared
-in include/eu3d.h (device context, step entry points) and include/eu3d_host.h
-(the reference-format readers and the Euler driver).  This module only binds
+in include/eu3d.h (device context, step entry points), include/eu3d_host.h
+(the reference-format readers and the Euler driver) and
+include/eu3d_surrogate.h (the learned chemistry surrogate).  This module only binds
 them with ctypes; there is no Python or CPU fallback for any compute entry
 point: if the library is missing, load() raises.
 """
@@ -25,6 +26,7 @@ SCHEME_TVD_RK3 = 1
 CHEM_COMPACT = 0x1
 CHEM_SKIP = 0x2
 CHEM_KEEP_CMS = 0x4
+CHEM_SURROGATE = 0x8
 OPT_GENERIC_MECHANISM = 1
 OPT_GENERATED_AVAILABLE = 2
 
@@ -63,7 +65,21 @@ class Timing(ctypes.Structure):
                                                  "cfl_calls"]]
 
 
-# name -> (restype, argtypes); every symbol declared in include/eu3d.h and include/eu3d_host.h
+class SurrogateParams(ctypes.Structure):
+    _fields_ = [("min_nchem", _i), ("tol_T", _d), ("tol_elem", _d), ("y_floor", _d),
+                ("hull_lo", ctypes.c_float * 32), ("hull_hi", ctypes.c_float * 32), ("proj", _d * 256)]
+
+
+class SurrogateStats(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_longlong) for n in ["routed", "fallback", "fail_hull", "fail_elem", "fail_neg",
+                                                 "fail_T"]]
+
+
+_ll = ctypes.c_longlong
+_f = ctypes.c_float
+
+# name -> (restype, argtypes); every symbol declared in include/eu3d.h,
+# include/eu3d_host.h and include/eu3d_surrogate.h
 SIGNATURES = {
     "eu3d_create": (_i, [ctypes.POINTER(Mesh), ctypes.POINTER(Mechanism), ctypes.POINTER(_vp)]),
     "eu3d_destroy": (_i, [_vp]),
@@ -101,6 +117,23 @@ SIGNATURES = {
     "eu3d_solver_nchem": (ctypes.POINTER(_i), [_vp]),
     "eu3d_solver_dump": (_i, [_vp, ctypes.c_char_p, _i]),
     "eu3d_solver_error": (ctypes.c_char_p, [_vp]),
+    "eu3d_mlp_create": (_i, [_i, _i, _i, _i, ctypes.c_ulonglong, ctypes.POINTER(_vp)]),
+    "eu3d_mlp_destroy": (_i, [_vp]),
+    "eu3d_mlp_last_error": (ctypes.c_char_p, [_vp]),
+    "eu3d_mlp_shape": (_i, [_vp, _vp, _vp, _vp, _vp]),
+    "eu3d_mlp_set_norm": (_i, [_vp, _vp, _vp, _vp, _vp]),
+    "eu3d_mlp_get_norm": (_i, [_vp, _vp, _vp, _vp, _vp]),
+    "eu3d_mlp_param_count": (_ll, [_vp]),
+    "eu3d_mlp_get_params": (_i, [_vp, _vp]),
+    "eu3d_mlp_set_params": (_i, [_vp, _vp]),
+    "eu3d_mlp_gradients": (_i, [_vp, _vp, _vp, _i, _vp, ctypes.POINTER(_d)]),
+    "eu3d_mlp_predict": (_i, [_vp, _vp, _ll, _vp]),
+    "eu3d_mlp_set_dataset": (_i, [_vp, _vp, _vp, _ll]),
+    "eu3d_mlp_train": (_i, [_vp, _i, _i, _f, _f, ctypes.c_ulonglong, _vp]),
+    "eu3d_set_surrogate": (_i, [_vp, _vp, ctypes.POINTER(SurrogateParams)]),
+    "eu3d_surrogate_stats_get": (_i, [_vp, ctypes.POINTER(SurrogateStats)]),
+    "eu3d_set_cells": (_i, [_vp, _ll, _vp, _vp, _vp]),
+    "eu3d_chemistry_harvest": (_i, [_vp, _d, _i, _i, _ll, _vp, _vp, ctypes.POINTER(_ll)]),
 }
 
 _lib = None

@@ -3,7 +3,8 @@
     python eu3d-gpu_amd/build.py
 
 produces, in this directory:
-  libeu3d.so   -- HIP kernels + the C ABI (include/eu3d.h, include/eu3d_host.h)
+  libeu3d.so   -- HIP kernels + the C ABI (include/eu3d.h, include/eu3d_host.h,
+                  include/eu3d_surrogate.h)
   host/eu3d_run -- the command-line driver
 Objects go to _build/ (not committed).
 """
@@ -26,7 +27,7 @@ HIP_FLAGS = ["--offload-arch=" + ARCH, "-O3", "-std=c++17", "-ffp-contract=off",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wall", "-Wno-unused-function"]
 CXX_FLAGS = ["-std=c++17", "-O3", "-ffp-contract=off", "-fno-fast-math", "-fPIC", "-Wall"]
 
-HIP_SOURCES = ["csrc/eu3d_hip.hip"]
+HIP_SOURCES = ["csrc/eu3d_hip.hip", "csrc/eu3d_mlp.hip"]
 HOST_SOURCES = ["host/eu3d_host.cpp", "host/host_abi.cpp"]
 
 

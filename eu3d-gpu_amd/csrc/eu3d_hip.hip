@@ -17,7 +17,9 @@
 #include <vector>
 
 #include "../../include/eu3d.h"
+#include "../../include/eu3d_surrogate.h"
 #include "eu3d_device.h"
+#include "eu3d_mlp_internal.h"
 
 using namespace eu3d;
 
@@ -35,6 +37,20 @@ struct ChemCounters {
     int max_nchem;
     int pad;
     unsigned long long cfl_bits;
+    // learned surrogate: routed cells, FP64 fallbacks and their causes
+    unsigned int sur_count;
+    unsigned int fb_count;
+    unsigned int fb_fetch;
+    unsigned int pad2;
+    unsigned long long fail_hull, fail_elem, fail_neg, fail_T;
+};
+
+// Acceptance parameters of the surrogate (eu3d_surrogate_params) on the device.
+struct SurDev {
+    int min_nchem, n_in, n_out, pad;
+    double tol_T, tol_elem, y_floor;
+    float lo[32], hi[32];
+    double proj[16 * 16];
 };
 
 struct TimedKind {
@@ -55,6 +71,13 @@ struct eu3d_ctx {
     double *prim, *T, *E, *C, *CS, *CS0, *CMS, *scratch;
     int *nchem, *queue;
     ChemCounters *cnt;
+    // learned surrogate (not owned) and its work lists
+    eu3d_mlp *mlp;
+    int sur_min;
+    SurDev *dsur;
+    int *sur_list, *fb_list;
+    unsigned char *sur_ok;
+    int sur_ok_cap;
     bool has_state, cms_valid;
     bool generated; // tables bit-identical to MechH2JA919: use the compile-time kernels
     bool generated_available;
@@ -503,6 +526,13 @@ __global__ void k_chem_reset(ChemCounters *c)
     c->stiff = 0;
     c->substeps = 0;
     c->max_nchem = 0;
+    c->sur_count = 0;
+    c->fb_count = 0;
+    c->fb_fetch = 0;
+    c->fail_hull = 0;
+    c->fail_elem = 0;
+    c->fail_neg = 0;
+    c->fail_T = 0;
 }
 
 template <int NS>
@@ -608,7 +638,7 @@ __device__ __forceinline__ unsigned long long lanemask_lt()
 template <int NS, bool FIRST, class MT>
 __device__ __forceinline__ int chem_step_cell(const Grid &g, const MT &m, long long c, double dtm, int n,
                                               double *CS, double *prim, double *T, double *E, double *C,
-                                              double *CMS, int keep_cms)
+                                              double *CMS, int keep_cms, int sur_min = 0)
 {
     const long long N = g.N;
     const double Tc = T[c];
@@ -632,6 +662,8 @@ __device__ __forceinline__ int chem_step_cell(const Grid &g, const MT &m, long l
         n = predict_nchem<NS>(R.W, Dcgs, Y, dtm);
         if (n < 1)
             return n;
+        if (sur_min > 0 && n >= sur_min) // left to the surrogate: no update here
+            return -n;
     }
     double cms[NS], cs[NS + 4];
     CellState<NS> st;
@@ -643,13 +675,14 @@ __device__ __forceinline__ int chem_step_cell(const Grid &g, const MT &m, long l
 // First chemistry pass over every interior cell: rates at the post-advection
 // state, Nchem (TrapezoidPrediction), and sub-step 0 with the same rates (the
 // reference evaluates them twice on identical inputs).  Cells with Nchem > 1
-// are appended to the stiff queue.
-template <int NS, class MT>
+// are appended to the stiff queue.  SUR: cells with Nchem >= sur_min are not
+// advanced; they are appended to sur_list for the network.
+template <int NS, class MT, bool SUR>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(EU3D_CHEM_WAVES_PER_EU)))
     k_chem_first(Grid g, const DevMech *__restrict__ mp, double dtm, double *__restrict__ CS,
                  double *__restrict__ prim, double *__restrict__ T, double *__restrict__ E, double *__restrict__ C,
                  double *__restrict__ CMS, int *__restrict__ nchem, int *__restrict__ queue, ChemCounters *cnt,
-                 int compact, int keep_cms)
+                 int compact, int keep_cms, int sur_min, int *__restrict__ sur_list)
 {
     const auto &m = mech_ref<MT>(mp);
     long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
@@ -657,14 +690,30 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(EU3
     bool active = interior_cell(g, t, i, j, k);
     long long c = active ? gidx(g, i, j, k) : 0;
     int n = 0;
-    bool stiff = false;
+    bool stiff = false, sur = false;
     if (active) {
-        n = chem_step_cell<NS, true>(g, m, c, dtm, 0, CS, prim, T, E, C, CMS, keep_cms);
+        n = chem_step_cell<NS, true>(g, m, c, dtm, 0, CS, prim, T, E, C, CMS, keep_cms, SUR ? sur_min : 0);
+        if (SUR && n < 0) {
+            n = -n;
+            sur = true;
+        }
         nchem[c] = n;
         if (n >= 1)
-            stiff = n > 1;
+            stiff = !sur && n > 1;
         else
             atomicAdd(&cnt->invalid, 1ull);
+    }
+    if (SUR) {
+        unsigned long long smask = __ballot(sur);
+        if (smask) {
+            unsigned int base = 0;
+            int leader = __ffsll((long long)smask) - 1;
+            if ((int)(threadIdx.x & 63) == leader)
+                base = atomicAdd(&cnt->sur_count, (unsigned int)__popcll(smask));
+            base = __shfl(base, leader, 64);
+            if (sur)
+                sur_list[base + __popcll(smask & lanemask_lt())] = (int)c;
+        }
     }
     // wave-aggregated append of stiff cells
     unsigned long long mask = __ballot(stiff);
@@ -741,6 +790,308 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(EU3
     int n = nchem[c];
     if (n > 1)
         chem_rest<NS>(g, mech_ref<MT>(mp), c, n, dtm, CS, prim, T, E, C, CMS, keep_cms);
+}
+
+// ---------------------------------------------------------------------------
+// learned surrogate inside the chemistry (no reference counterpart)
+// ---------------------------------------------------------------------------
+using eu3d_mlp_internal::bf16;
+using eu3d_mlp_internal::f2bf;
+using eu3d_mlp_internal::kIn;
+using eu3d_mlp_internal::kOut;
+
+// Raw features of cell c: T, ln p, ln(Y_s + 1e-12), ln dt.
+template <int NS>
+__device__ __forceinline__ void sur_features(long long N, long long c, double dt, const double *__restrict__ prim,
+                                             const double *__restrict__ T, float (&f)[NS + 3])
+{
+    f[0] = (float)T[c];
+    f[1] = (float)log(prim[c]);
+#pragma unroll
+    for (int s = 0; s < NS; s++)
+        f[2 + s] = (float)log(prim[(7 + s) * N + c] + 1e-12);
+    f[NS + 2] = (float)log(dt);
+}
+
+// Normalized BF16 network input of `count` listed cells; rows up to Mpad are
+// zero-filled.  ok[row] = all features inside the training box.
+template <int NS>
+__global__ void __launch_bounds__(kBlock)
+    k_sur_features(Grid g, const SurDev *__restrict__ sp, const int *__restrict__ list, int count, int Mpad,
+                   double dt, const double *__restrict__ prim, const double *__restrict__ T,
+                   const float *__restrict__ norm, bf16 *__restrict__ X0, unsigned char *__restrict__ ok)
+{
+    const int row = blockIdx.x * blockDim.x + threadIdx.x;
+    if (row >= Mpad)
+        return;
+    bf16 *xr = X0 + (long long)row * kIn;
+    if (row >= count) {
+        for (int q = 0; q < kIn; q++)
+            xr[q] = 0;
+        return;
+    }
+    float f[NS + 3];
+    sur_features<NS>(g.N, list[row], dt, prim, T, f);
+    bool in = true;
+#pragma unroll
+    for (int q = 0; q < NS + 3; q++)
+        in = in && f[q] >= sp->lo[q] && f[q] <= sp->hi[q]; // NaN fails
+#pragma unroll
+    for (int q = 0; q < NS + 3; q++)
+        xr[q] = f2bf((f[q] - norm[q]) / norm[kIn + q]);
+    for (int q = NS + 3; q < kIn; q++)
+        xr[q] = 0;
+    ok[row] = in ? 1 : 0;
+}
+
+// Apply the network's (dY, dT) to the listed cells at constant density,
+// momentum and internal energy, with the acceptance checks; a failing cell is
+// left unchanged and appended to fb_list for the FP64 integrator.
+template <int NS, class MT>
+__global__ void __launch_bounds__(kBlock)
+    k_sur_apply(Grid g, const DevMech *__restrict__ mp, const SurDev *__restrict__ sp, const int *__restrict__ list,
+                int count, double dt, const float *__restrict__ Y, const float *__restrict__ norm,
+                const unsigned char *__restrict__ ok, double *__restrict__ CS, double *__restrict__ prim,
+                double *__restrict__ T, double *__restrict__ E, double *__restrict__ C, double *__restrict__ CMS,
+                int keep_cms, int *__restrict__ fb_list, ChemCounters *cnt)
+{
+    const int row = blockIdx.x * blockDim.x + threadIdx.x;
+    if (row >= count)
+        return;
+    const auto &m = mech_ref<MT>(mp);
+    const long long N = g.N, c = list[row];
+    double rho[NS], D = 0.0;
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+        rho[s] = CS[s * N + c];
+        D += rho[s];
+    }
+    const double T0 = T[c], E0 = E[c];
+    const double U = prim[2 * N + c], V = prim[3 * N + c], W = prim[4 * N + c];
+    const float *yr = Y + (long long)row * kOut;
+    const float *om = norm + 2 * kIn, *os = norm + 2 * kIn + kOut;
+    double dY[NS];
+#pragma unroll
+    for (int s = 0; s < NS; s++)
+        dY[s] = (double)(yr[s] * os[s] + om[s]);
+    const double dT = (double)(yr[NS] * os[NS] + om[NS]);
+    int fail = ok[row] ? 0 : 1;
+    // element-conservation projection
+    double dYp[NS], corr = 0.0, mag = 0.0;
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+        double a = 0.0;
+#pragma unroll
+        for (int q = 0; q < NS; q++)
+            a += sp->proj[s * 16 + q] * dY[q];
+        dYp[s] = a;
+        corr += fabs(a - dY[s]);
+        mag += fabs(a);
+    }
+    if (!fail && !(corr <= sp->tol_elem * (mag + 1e-6)))
+        fail = 2;
+    double cs[NS + 4], Dn = 0.0;
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+        double y = rho[s] / D + dYp[s];
+        if (!fail && !(y >= -sp->y_floor))
+            fail = 3;
+        y = fmax(y, 0.0);
+        cs[s] = D * y;
+        Dn += cs[s];
+    }
+    cs[NS] = U * Dn;
+    cs[NS + 1] = V * Dn;
+    cs[NS + 2] = W * Dn;
+    cs[NS + 3] = Dn * E0 + 0.5 * ((U * U) + (V * V) + (W * W)) * Dn;
+    CellState<NS> st;
+    if (!fail) {
+        double Tg = T0 + dT;
+        if (!(Tg > 200.0 && Tg < 6000.0))
+            Tg = T0;
+        recover<NS>(m, cs, Tg, E0, st);
+        if (!(isfinite(st.T) && st.T > 0.0 && fabs(st.T - (T0 + dT)) <= sp->tol_T))
+            fail = 4;
+    }
+    if (fail) {
+        unsigned long long *why = fail == 1 ? &cnt->fail_hull
+                                  : fail == 2 ? &cnt->fail_elem
+                                  : fail == 3 ? &cnt->fail_neg
+                                              : &cnt->fail_T;
+        atomicAdd(why, 1ull);
+        fb_list[atomicAdd(&cnt->fb_count, 1u)] = (int)c;
+        return;
+    }
+#pragma unroll
+    for (int q = 0; q < NS + 4; q++)
+        CS[q * N + c] = cs[q];
+    if (keep_cms) {
+#pragma unroll
+        for (int s = 0; s < NS; s++)
+            CMS[s * N + c] = (cs[s] - rho[s]) / dt;
+    }
+    store_state<NS>(N, c, st, prim, T, E, C);
+}
+
+// FP64 integration (all sub-steps) of the cells the surrogate rejected.
+template <int NS, class MT>
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(EU3D_CHEM_WAVES_PER_EU)))
+    k_chem_fallback(Grid g, const DevMech *__restrict__ mp, double dtm, double *__restrict__ CS,
+                    double *__restrict__ prim, double *__restrict__ T, double *__restrict__ E,
+                    double *__restrict__ C, double *__restrict__ CMS, const int *__restrict__ fb_list,
+                    ChemCounters *cnt, int keep_cms)
+{
+    const unsigned int total = cnt->fb_count;
+    const int lane = threadIdx.x & 63;
+    for (;;) {
+        unsigned int base = 0;
+        if (lane == 0)
+            base = atomicAdd(&cnt->fb_fetch, 64u);
+        base = __shfl(base, 0, 64);
+        if (base >= total)
+            break;
+        unsigned int q = base + lane;
+        if (q < total) {
+            const auto &m = mech_ref<MT>(mp);
+            long long c = fb_list[q];
+            int n = chem_step_cell<NS, true>(g, m, c, dtm, 0, CS, prim, T, E, C, CMS, keep_cms);
+            if (n > 1)
+                chem_rest<NS>(g, m, c, n, dtm, CS, prim, T, E, C, CMS, keep_cms);
+        }
+    }
+}
+
+// Cells at rest from (T, p, Y), as FieldInitial builds them
+// (src/Flowfield.cpp:193-273): rho = p / (Rgas T), E = sum Y (h - R_s T).
+template <int NS, class MT>
+__global__ void __launch_bounds__(kBlock)
+    k_set_cells(Grid g, const DevMech *__restrict__ mp, long long n, const double *__restrict__ Tin,
+                const double *__restrict__ pin, const double *__restrict__ Yin, double *__restrict__ CS,
+                double *__restrict__ prim, double *__restrict__ T, double *__restrict__ E, double *__restrict__ C,
+                double *__restrict__ CMS)
+{
+    long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    int i, j, k;
+    if (t >= n || !interior_cell(g, t, i, j, k))
+        return;
+    const auto &m = mech_ref<MT>(mp);
+    const long long N = g.N, c = gidx(g, i, j, k);
+    const double Tc = Tin[t], p = pin[t];
+    double Y[NS], sYM = 0.0;
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+        Y[s] = Yin[t * NS + s];
+        sYM += Y[s] / m.mw[s];
+    }
+    const double Wav = 1.0 / sYM, Rgas = m.R * 1000 / Wav, D = p / (Rgas * Tc);
+    TPow tp = tpow(Tc);
+    double e = 0.0;
+    by_range(m, Tc, [&](const auto &a) {
+#pragma unroll
+        for (int s = 0; s < NS; s++)
+            e += Y[s] * (gethi(tp, m.ri[s], a, s, NS) - m.ri[s] * Tc);
+    });
+    const double Ei = e * 1000;
+    double cs[NS + 4];
+#pragma unroll
+    for (int s = 0; s < NS; s++)
+        cs[s] = D * Y[s];
+    cs[NS] = 0.0;
+    cs[NS + 1] = 0.0;
+    cs[NS + 2] = 0.0;
+    cs[NS + 3] = D * Ei;
+    CellState<NS> st;
+    recover<NS>(m, cs, Tc, Ei, st);
+#pragma unroll
+    for (int q = 0; q < NS + 4; q++)
+        CS[q * N + c] = cs[q];
+#pragma unroll
+    for (int s = 0; s < NS; s++)
+        CMS[s * N + c] = 0.0;
+    store_state<NS>(N, c, st, prim, T, E, C);
+}
+
+// Harvest, before the chemistry: features (float) and the old Y, T (double)
+// of every interior cell, at its interior index.
+template <int NS>
+__global__ void __launch_bounds__(kBlock)
+    k_harvest_pre(Grid g, double dt, const double *__restrict__ prim, const double *__restrict__ T,
+                  float *__restrict__ hx, double *__restrict__ hold)
+{
+    long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    int i, j, k;
+    if (!interior_cell(g, t, i, j, k))
+        return;
+    const long long N = g.N, c = gidx(g, i, j, k);
+    float f[NS + 3];
+    sur_features<NS>(N, c, dt, prim, T, f);
+#pragma unroll
+    for (int q = 0; q < NS + 3; q++)
+        hx[t * (NS + 3) + q] = f[q];
+#pragma unroll
+    for (int s = 0; s < NS; s++)
+        hold[t * (NS + 1) + s] = prim[(7 + s) * N + c];
+    hold[t * (NS + 1) + NS] = T[c];
+}
+
+__device__ __forceinline__ bool harvest_pick(const Grid &g, long long t, const int *nchem, int min_nchem)
+{
+    int i, j, k;
+    if (!interior_cell(g, t, i, j, k))
+        return false;
+    return nchem[gidx(g, i, j, k)] >= min_nchem;
+}
+
+// Harvest, after the chemistry: number of picked cells per block of kBlock.
+__global__ void __launch_bounds__(kBlock)
+    k_harvest_count(Grid g, const int *__restrict__ nchem, int min_nchem, unsigned int *__restrict__ counts)
+{
+    __shared__ unsigned int tot;
+    if (threadIdx.x == 0)
+        tot = 0;
+    __syncthreads();
+    long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    unsigned long long mask = __ballot(harvest_pick(g, t, nchem, min_nchem));
+    if ((threadIdx.x & 63) == 0 && mask)
+        atomicAdd(&tot, (unsigned int)__popcll(mask));
+    __syncthreads();
+    if (threadIdx.x == 0)
+        counts[blockIdx.x] = tot;
+}
+
+// Harvest: picked cells to rows offsets[block] + rank, in interior order.
+template <int NS>
+__global__ void __launch_bounds__(kBlock)
+    k_harvest_scatter(Grid g, const int *__restrict__ nchem, int min_nchem, const long long *__restrict__ offsets,
+                      long long max_rows, const float *__restrict__ hx, const double *__restrict__ hold,
+                      const double *__restrict__ prim, const double *__restrict__ T, float *__restrict__ ox,
+                      float *__restrict__ ot)
+{
+    __shared__ unsigned int wsum[kBlock / 64];
+    long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    const bool pick = harvest_pick(g, t, nchem, min_nchem);
+    unsigned long long mask = __ballot(pick);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0)
+        wsum[w] = (unsigned int)__popcll(mask);
+    __syncthreads();
+    if (!pick)
+        return;
+    long long row = offsets[blockIdx.x] + __popcll(mask & lanemask_lt());
+    for (int q = 0; q < w; q++)
+        row += wsum[q];
+    if (row >= max_rows)
+        return;
+    int i, j, k;
+    interior_cell(g, t, i, j, k);
+    const long long N = g.N, c = gidx(g, i, j, k);
+#pragma unroll
+    for (int q = 0; q < NS + 3; q++)
+        ox[row * (NS + 3) + q] = hx[t * (NS + 3) + q];
+#pragma unroll
+    for (int s = 0; s < NS; s++)
+        ot[row * (NS + 1) + s] = (float)(prim[(7 + s) * N + c] - hold[t * (NS + 1) + s]);
+    ot[row * (NS + 1) + NS] = (float)(T[c] - hold[t * (NS + 1) + NS]);
 }
 
 // ---------------------------------------------------------------------------
@@ -883,9 +1234,9 @@ static void launch_chem(eu3d_ctx *ctx, double dt, int flags)
     int compact = (flags & EU3D_CHEM_COMPACT) ? 1 : 0;
     int keep = (flags & EU3D_CHEM_KEEP_CMS) ? 1 : 0;
     k_chem_reset<<<1, 1, 0, ctx->stream>>>(ctx->cnt);
-    k_chem_first<NS, MT><<<blocks_for(ctx->n_int), kBlock, 0, ctx->stream>>>(
+    k_chem_first<NS, MT, false><<<blocks_for(ctx->n_int), kBlock, 0, ctx->stream>>>(
         g, ctx->mech, dt, ctx->CS, ctx->prim, ctx->T, ctx->E, ctx->C, ctx->CMS, ctx->nchem, ctx->queue, ctx->cnt,
-        compact, keep);
+        compact, keep, 0, nullptr);
     if (compact) {
         k_chem_queue<NS, MT><<<ctx->ncu * 4, kBlock, 0, ctx->stream>>>(g, ctx->mech, dt, ctx->CS, ctx->prim, ctx->T, ctx->E,
                                                               ctx->C, ctx->CMS, ctx->nchem, ctx->queue, ctx->cnt,
@@ -895,6 +1246,73 @@ static void launch_chem(eu3d_ctx *ctx, double dt, int flags)
                                                                            ctx->T, ctx->E, ctx->C, ctx->CMS,
                                                                            ctx->nchem, keep);
     }
+}
+
+// Chemistry with the surrogate: cells with Nchem >= sur_min go through the
+// network in batches of its max_batch rows; the rejected ones and the other
+// stiff cells are integrated in FP64 afterwards.
+template <int NS, class MT>
+static void launch_chem_sur(eu3d_ctx *ctx, double dt, int flags, int *rc)
+{
+    const Grid &g = ctx->g;
+    int keep = (flags & EU3D_CHEM_KEEP_CMS) ? 1 : 0;
+    k_chem_reset<<<1, 1, 0, ctx->stream>>>(ctx->cnt);
+    k_chem_first<NS, MT, true><<<blocks_for(ctx->n_int), kBlock, 0, ctx->stream>>>(
+        g, ctx->mech, dt, ctx->CS, ctx->prim, ctx->T, ctx->E, ctx->C, ctx->CMS, ctx->nchem, ctx->queue, ctx->cnt, 1,
+        keep, ctx->sur_min, ctx->sur_list);
+    unsigned int nsur = 0;
+    hipError_t e = hipMemcpyAsync(&nsur, &ctx->cnt->sur_count, sizeof nsur, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess)
+        e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) {
+        ctx->err = std::string("surrogate count: ") + hipGetErrorString(e);
+        *rc = EU3D_ERR_HIP;
+        return;
+    }
+    const eu3d_mlp_internal::View v = eu3d_mlp_internal::view(ctx->mlp);
+    for (unsigned int off = 0; off < nsur; off += (unsigned int)v.max_batch) {
+        const int count = (int)std::min<unsigned int>((unsigned int)v.max_batch, nsur - off);
+        const int Mpad = (count + 63) / 64 * 64;
+        k_sur_features<NS><<<blocks_for(Mpad), kBlock, 0, ctx->stream>>>(g, ctx->dsur, ctx->sur_list + off, count,
+                                                                         Mpad, dt, ctx->prim, ctx->T, v.norm, v.X0,
+                                                                         ctx->sur_ok);
+        int r = eu3d_mlp_internal::forward(ctx->mlp, ctx->stream, Mpad);
+        if (r) {
+            ctx->err = std::string("surrogate forward: ") + eu3d_mlp_last_error(ctx->mlp);
+            *rc = r;
+            return;
+        }
+        k_sur_apply<NS, MT><<<blocks_for(count), kBlock, 0, ctx->stream>>>(
+            g, ctx->mech, ctx->dsur, ctx->sur_list + off, count, dt, v.Y, v.norm, ctx->sur_ok, ctx->CS, ctx->prim,
+            ctx->T, ctx->E, ctx->C, ctx->CMS, keep, ctx->fb_list, ctx->cnt);
+    }
+    k_chem_queue<NS, MT><<<ctx->ncu * 4, kBlock, 0, ctx->stream>>>(g, ctx->mech, dt, ctx->CS, ctx->prim, ctx->T,
+                                                                   ctx->E, ctx->C, ctx->CMS, ctx->nchem, ctx->queue,
+                                                                   ctx->cnt, keep);
+    k_chem_fallback<NS, MT><<<ctx->ncu * 4, kBlock, 0, ctx->stream>>>(g, ctx->mech, dt, ctx->CS, ctx->prim, ctx->T,
+                                                                      ctx->E, ctx->C, ctx->CMS, ctx->fb_list,
+                                                                      ctx->cnt, keep);
+}
+
+template <int NS, class MT>
+static void launch_set_cells(eu3d_ctx *ctx, long long n, const double *T, const double *p, const double *Y)
+{
+    k_set_cells<NS, MT><<<blocks_for(n), kBlock, 0, ctx->stream>>>(ctx->g, ctx->mech, n, T, p, Y, ctx->CS, ctx->prim,
+                                                                   ctx->T, ctx->E, ctx->C, ctx->CMS);
+}
+
+template <int NS, class MT>
+static void launch_harvest_pre(eu3d_ctx *ctx, double dt, float *hx, double *hold)
+{
+    k_harvest_pre<NS><<<blocks_for(ctx->n_int), kBlock, 0, ctx->stream>>>(ctx->g, dt, ctx->prim, ctx->T, hx, hold);
+}
+
+template <int NS, class MT>
+static void launch_harvest_scatter(eu3d_ctx *ctx, int min_nchem, const long long *offsets, long long max_rows,
+                                   const float *hx, const double *hold, float *ox, float *ot)
+{
+    k_harvest_scatter<NS><<<blocks_for(ctx->n_int), kBlock, 0, ctx->stream>>>(
+        ctx->g, ctx->nchem, min_nchem, offsets, max_rows, hx, hold, ctx->prim, ctx->T, ox, ot);
 }
 
 template <int NS, class MT>
@@ -980,8 +1398,10 @@ int eu3d_destroy(eu3d_ctx *ctx)
     resolve_timing(ctx);
     for (auto e : ctx->free_events)
         (void)hipEventDestroy(e);
-    void *ptrs[] = {ctx->mech, ctx->x,   ctx->y,   ctx->z,       ctx->prim,  ctx->T,     ctx->E,  ctx->C,
-                    ctx->CS,   ctx->CS0, ctx->CMS, ctx->scratch, ctx->nchem, ctx->queue, ctx->cnt};
+    void *ptrs[] = {ctx->mech,  ctx->x,      ctx->y,     ctx->z,       ctx->prim,    ctx->T,
+                    ctx->E,     ctx->C,      ctx->CS,    ctx->CS0,     ctx->CMS,     ctx->scratch,
+                    ctx->nchem, ctx->queue,  ctx->cnt,   ctx->dsur,    ctx->sur_list, ctx->fb_list,
+                    ctx->sur_ok};
     for (void *p : ptrs)
         if (p)
             (void)hipFree(p);
@@ -1317,9 +1737,191 @@ int eu3d_chemistry(eu3d_ctx *ctx, double dt, int flags)
         FAIL(EU3D_ERR_ARG, "dt must be > 0");
     if (flags & EU3D_CHEM_SKIP)
         return EU3D_OK;
-    Timer tm(ctx, K_CHEM);
-    EU3D_DISPATCH_NS(ctx->ns, launch_chem, ctx, dt, flags);
+    if (flags & EU3D_CHEM_SURROGATE) {
+        if (!ctx->mlp)
+            FAIL(EU3D_ERR_STATE, "EU3D_CHEM_SURROGATE without an attached network (eu3d_set_surrogate)");
+        if (!(flags & EU3D_CHEM_COMPACT))
+            FAIL(EU3D_ERR_ARG, "EU3D_CHEM_SURROGATE requires EU3D_CHEM_COMPACT");
+        Timer tm(ctx, K_CHEM);
+        int rc = EU3D_OK;
+        EU3D_DISPATCH_NS(ctx->ns, launch_chem_sur, ctx, dt, flags, &rc);
+        if (rc)
+            return rc;
+    } else {
+        Timer tm(ctx, K_CHEM);
+        EU3D_DISPATCH_NS(ctx->ns, launch_chem, ctx, dt, flags);
+    }
     CK(hipGetLastError());
+    return EU3D_OK;
+}
+
+int eu3d_set_surrogate(eu3d_ctx *ctx, eu3d_mlp *mlp, const eu3d_surrogate_params *p)
+{
+    if (!ctx)
+        return EU3D_ERR_ARG;
+    if (!mlp) {
+        ctx->mlp = nullptr;
+        return EU3D_OK;
+    }
+    if (!p)
+        FAIL(EU3D_ERR_ARG, "surrogate parameters missing");
+    int n_in = 0, n_out = 0, max_batch = 0;
+    eu3d_mlp_shape(mlp, &n_in, nullptr, &n_out, &max_batch);
+    if (n_in != ctx->ns + 3 || n_out != ctx->ns + 1)
+        FAIL(EU3D_ERR_ARG, "network shape must be ns+3 -> ns+1");
+    if (p->min_nchem < 1 || !(p->tol_T > 0) || !(p->tol_elem > 0) || !(p->y_floor >= 0))
+        FAIL(EU3D_ERR_ARG, "surrogate parameters out of range");
+    SurDev h{};
+    h.min_nchem = p->min_nchem;
+    h.n_in = n_in;
+    h.n_out = n_out;
+    h.tol_T = p->tol_T;
+    h.tol_elem = p->tol_elem;
+    h.y_floor = p->y_floor;
+    for (int q = 0; q < 32; q++) {
+        h.lo[q] = p->hull_lo[q];
+        h.hi[q] = p->hull_hi[q];
+    }
+    for (int q = 0; q < 16 * 16; q++)
+        h.proj[q] = p->proj[q];
+    CK(hipStreamSynchronize(ctx->stream));
+    if (!ctx->dsur)
+        CK(hipMalloc(&ctx->dsur, sizeof(SurDev)));
+    CK(hipMemcpy(ctx->dsur, &h, sizeof h, hipMemcpyHostToDevice));
+    if (!ctx->sur_list)
+        CK(hipMalloc(&ctx->sur_list, sizeof(int) * ctx->n_int));
+    if (!ctx->fb_list)
+        CK(hipMalloc(&ctx->fb_list, sizeof(int) * ctx->n_int));
+    if (ctx->sur_ok_cap < max_batch) {
+        if (ctx->sur_ok)
+            CK(hipFree(ctx->sur_ok));
+        ctx->sur_ok = nullptr;
+        CK(hipMalloc(&ctx->sur_ok, max_batch));
+        ctx->sur_ok_cap = max_batch;
+    }
+    ctx->sur_min = p->min_nchem;
+    ctx->mlp = mlp;
+    return EU3D_OK;
+}
+
+int eu3d_surrogate_stats_get(eu3d_ctx *ctx, eu3d_surrogate_stats *out)
+{
+    if (!ctx || !out)
+        return EU3D_ERR_ARG;
+    ChemCounters c;
+    CK(hipMemcpyAsync(&c, ctx->cnt, sizeof c, hipMemcpyDeviceToHost, ctx->stream));
+    CK(hipStreamSynchronize(ctx->stream));
+    out->routed = (long long)c.sur_count;
+    out->fallback = (long long)c.fb_count;
+    out->fail_hull = (long long)c.fail_hull;
+    out->fail_elem = (long long)c.fail_elem;
+    out->fail_neg = (long long)c.fail_neg;
+    out->fail_T = (long long)c.fail_T;
+    return EU3D_OK;
+}
+
+int eu3d_set_cells(eu3d_ctx *ctx, long long n, const double *T, const double *p, const double *Y)
+{
+    if (!ctx || !T || !p || !Y || n < 1 || n > ctx->n_int)
+        return EU3D_ERR_ARG;
+    if (!ctx->has_state)
+        FAIL(EU3D_ERR_STATE, "no state uploaded");
+    const int ns = ctx->ns;
+    for (long long t = 0; t < n; t++) {
+        if (!(T[t] > 0) || !(p[t] > 0))
+            FAIL(EU3D_ERR_ARG, "T and p must be > 0");
+        for (int s = 0; s < ns; s++)
+            if (!(Y[t * ns + s] >= 0))
+                FAIL(EU3D_ERR_ARG, "mass fractions must be >= 0");
+    }
+    double *d = nullptr;
+    CK(hipMalloc(&d, sizeof(double) * n * (ns + 2)));
+    hipError_t e = hipMemcpy(d, T, sizeof(double) * n, hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        e = hipMemcpy(d + n, p, sizeof(double) * n, hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        e = hipMemcpy(d + 2 * n, Y, sizeof(double) * n * ns, hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+        EU3D_DISPATCH_NS(ns, launch_set_cells, ctx, n, d, d + n, d + 2 * n);
+        e = hipStreamSynchronize(ctx->stream);
+    }
+    (void)hipFree(d);
+    if (e != hipSuccess) {
+        ctx->err = std::string("eu3d_set_cells: ") + hipGetErrorString(e);
+        return EU3D_ERR_HIP;
+    }
+    return EU3D_OK;
+}
+
+int eu3d_chemistry_harvest(eu3d_ctx *ctx, double dt, int flags, int min_nchem, long long max_rows, float *x,
+                           float *t, long long *rows)
+{
+    if (!ctx || !rows || min_nchem < 1 || max_rows < 0 || (max_rows > 0 && (!x || !t)))
+        return EU3D_ERR_ARG;
+    if (flags & (EU3D_CHEM_SURROGATE | EU3D_CHEM_SKIP))
+        FAIL(EU3D_ERR_ARG, "harvest runs the FP64 chemistry (no SURROGATE / SKIP)");
+    if (!ctx->has_state)
+        FAIL(EU3D_ERR_STATE, "no state uploaded");
+    const int ns = ctx->ns;
+    const long long n = ctx->n_int;
+    const unsigned int nb = blocks_for(n);
+    float *hx = nullptr, *ox = nullptr;
+    double *hold = nullptr;
+    unsigned int *counts = nullptr;
+    long long *offs = nullptr;
+    int rc = EU3D_OK;
+    hipError_t e = hipMalloc(&hx, sizeof(float) * n * (ns + 3));
+    if (e == hipSuccess)
+        e = hipMalloc(&hold, sizeof(double) * n * (ns + 1));
+    if (e == hipSuccess)
+        e = hipMalloc(&counts, sizeof(unsigned int) * nb);
+    if (e == hipSuccess)
+        e = hipMalloc(&offs, sizeof(long long) * nb);
+    if (e == hipSuccess && max_rows > 0)
+        e = hipMalloc(&ox, sizeof(float) * max_rows * (2 * ns + 4));
+    std::vector<unsigned int> hc(nb);
+    std::vector<long long> ho(nb);
+    long long total = 0;
+    if (e == hipSuccess) {
+        EU3D_DISPATCH_NS(ns, launch_harvest_pre, ctx, dt, hx, hold);
+        rc = eu3d_chemistry(ctx, dt, flags);
+    }
+    if (e == hipSuccess && rc == EU3D_OK) {
+        k_harvest_count<<<nb, kBlock, 0, ctx->stream>>>(ctx->g, ctx->nchem, min_nchem, counts);
+        e = hipMemcpyAsync(hc.data(), counts, sizeof(unsigned int) * nb, hipMemcpyDeviceToHost, ctx->stream);
+        if (e == hipSuccess)
+            e = hipStreamSynchronize(ctx->stream);
+    }
+    if (e == hipSuccess && rc == EU3D_OK) {
+        for (unsigned int b = 0; b < nb; b++) {
+            ho[b] = total;
+            total += hc[b];
+        }
+        long long w = std::min(total, max_rows);
+        if (w > 0) {
+            e = hipMemcpyAsync(offs, ho.data(), sizeof(long long) * nb, hipMemcpyHostToDevice, ctx->stream);
+            if (e == hipSuccess) {
+                float *ot = ox + max_rows * (ns + 3);
+                EU3D_DISPATCH_NS(ns, launch_harvest_scatter, ctx, min_nchem, offs, max_rows, hx, hold, ox, ot);
+                e = hipMemcpyAsync(x, ox, sizeof(float) * w * (ns + 3), hipMemcpyDeviceToHost, ctx->stream);
+                if (e == hipSuccess)
+                    e = hipMemcpyAsync(t, ot, sizeof(float) * w * (ns + 1), hipMemcpyDeviceToHost, ctx->stream);
+                if (e == hipSuccess)
+                    e = hipStreamSynchronize(ctx->stream);
+            }
+        }
+    }
+    void *bufs[] = {hx, hold, counts, offs, ox};
+    for (void *b : bufs)
+        if (b)
+            (void)hipFree(b);
+    if (rc)
+        return rc;
+    if (e != hipSuccess) {
+        ctx->err = std::string("eu3d_chemistry_harvest: ") + hipGetErrorString(e);
+        return EU3D_ERR_HIP;
+    }
+    *rows = total;
     return EU3D_OK;
 }
 

@@ -1,0 +1,626 @@
+// eu3d_mlp.hip -- the learned chemistry surrogate's network on gfx950:
+// an MLP (n_in -> H -> H -> H -> n_out, GELU) with BF16 weights and
+// activations, FP32 accumulation on the matrix cores (MFMA), FP32 master
+// weights and Adam.  Training runs entirely on the device; forward and
+// backward are hand-written MFMA GEMMs with fused epilogues.
+//
+// Shapes are padded: inputs to kIn = 32 columns, outputs to kOut = 32.  The
+// padded inputs are 0 and the padded targets give a zero loss gradient, so
+// the padded weights get zero gradients and stay 0 under Adam.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/eu3d.h"
+#include "../../include/eu3d_surrogate.h"
+#include "eu3d_mlp_internal.h"
+
+using eu3d_mlp_internal::bf16;
+using eu3d_mlp_internal::bf2f;
+using eu3d_mlp_internal::f2bf;
+using eu3d_mlp_internal::kIn;
+using eu3d_mlp_internal::kOut;
+
+namespace {
+
+constexpr int kLayers = 4;
+
+typedef short s16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+__device__ __forceinline__ float gelu(float x) { return 0.5f * x * (1.0f + erff(x * 0.70710678118654752f)); }
+__device__ __forceinline__ float gelu_grad(float x)
+{
+    return 0.5f * (1.0f + erff(x * 0.70710678118654752f)) + x * 0.39894228040143268f * expf(-0.5f * x * x);
+}
+
+enum Epi {
+    EPI_BIAS_GELU = 0, // Z = acc + b (fp32), H = gelu(Z) (bf16)
+    EPI_BIAS = 1,      // Y = acc + b (fp32)
+    EPI_GELU_BWD = 2,  // dZ = acc * gelu'(Zaux) (bf16)
+    EPI_ATOMIC = 3,    // C += acc (fp32, split over the reduction)
+    EPI_GELU_ONLY = 4, // H = gelu(acc + b) (bf16), inference: Z not kept
+};
+
+// C[M x N] = A[M x K] . B[K x N] with general strides, bf16 operands, fp32
+// accumulation on v_mfma_f32_32x32x16_bf16.  A wave computes a 32 x 32 tile;
+// 4 waves per block (64 x 64).  M, N multiples of 32; K (per split) multiple
+// of 16.  blockIdx.z splits the reduction (EPI_ATOMIC only).
+// Operand layout of the instruction: lane l holds A[l%32][8*(l/32)+t] and
+// B[8*(l/32)+t][l%32], t = 0..7; accumulator v of lane l is
+// C[8*(v/4) + 4*(l/32) + v%4][l%32].
+template <int EPI>
+__global__ void __launch_bounds__(256)
+    k_gemm(int M, int N, int K, const bf16 *__restrict__ A, long long sAm, long long sAk, const bf16 *__restrict__ B,
+           long long sBk, long long sBn, const float *__restrict__ bias, float *__restrict__ Cf,
+           bf16 *__restrict__ Cb, long long ldc, const float *__restrict__ Zaux)
+{
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    const int m0 = blockIdx.x * 64 + (w & 1) * 32;
+    const int n0 = blockIdx.y * 64 + (w >> 1) * 32;
+    if (m0 >= M || n0 >= N)
+        return;
+    const int kspl = K / gridDim.z;
+    const int kbeg = blockIdx.z * kspl, kend = kbeg + kspl;
+    const int r = l & 31, q = l >> 5;
+    f32x16 acc;
+#pragma unroll
+    for (int v = 0; v < 16; v++)
+        acc[v] = 0.f;
+    const bf16 *Ap = A + (long long)(m0 + r) * sAm;
+    const bf16 *Bp = B + (long long)(n0 + r) * sBn;
+    for (int k0 = kbeg; k0 < kend; k0 += 16) {
+        s16x8 a, b;
+#pragma unroll
+        for (int t = 0; t < 8; t++) {
+            long long k = k0 + 8 * q + t;
+            a[t] = (short)Ap[k * sAk];
+            b[t] = (short)Bp[k * sBk];
+        }
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+    }
+#pragma unroll
+    for (int v = 0; v < 16; v++) {
+        const int m = m0 + 8 * (v >> 2) + 4 * q + (v & 3);
+        const int n = n0 + r;
+        const long long o = (long long)m * ldc + n;
+        float x = acc[v];
+        if (EPI == EPI_BIAS_GELU) {
+            float z = x + bias[n];
+            Cf[o] = z;
+            Cb[o] = f2bf(gelu(z));
+        } else if (EPI == EPI_BIAS) {
+            Cf[o] = x + bias[n];
+        } else if (EPI == EPI_GELU_ONLY) {
+            Cb[o] = f2bf(gelu(x + bias[n]));
+        } else if (EPI == EPI_GELU_BWD) {
+            Cb[o] = f2bf(x * gelu_grad(Zaux[o]));
+        } else {
+            atomicAdd(&Cf[o], x);
+        }
+    }
+}
+
+// features (raw, n_in per row, gathered through idx) -> normalized bf16 [B x kIn];
+// targets -> normalized fp32 [B x kOut] (padding zero)
+__global__ void k_prep(int B, int n_in, int n_out, const long long *__restrict__ idx, const float *__restrict__ x,
+                       const float *__restrict__ t, const float *__restrict__ norm, bf16 *__restrict__ X0,
+                       float *__restrict__ T0)
+{
+    int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= B)
+        return;
+    long long s = idx ? idx[b] : b;
+    const float *im = norm, *is = norm + kIn, *om = norm + 2 * kIn, *os = norm + 2 * kIn + kOut;
+    for (int c = 0; c < kIn; c++)
+        X0[(long long)b * kIn + c] = c < n_in ? f2bf((x[s * n_in + c] - im[c]) / is[c]) : (bf16)0;
+    if (t)
+        for (int c = 0; c < kOut; c++)
+            T0[(long long)b * kOut + c] = c < n_out ? (t[s * n_out + c] - om[c]) / os[c] : 0.f;
+}
+
+// MSE over the n_out real outputs; dY = 2 (y - t) / (B n_out) as bf16
+__global__ void k_loss(int B, int n_out, const float *__restrict__ Y, const float *__restrict__ T0,
+                       bf16 *__restrict__ dY, double *loss)
+{
+    int b = blockIdx.x * blockDim.x + threadIdx.x;
+    double part = 0.0;
+    if (b < B) {
+        for (int c = 0; c < kOut; c++) {
+            long long o = (long long)b * kOut + c;
+            float d = c < n_out ? Y[o] - T0[o] : 0.f;
+            part += (double)d * d;
+            dY[o] = f2bf(2.0f * d / ((float)B * n_out));
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1)
+        part += __shfl_xor(part, off, 64);
+    if ((threadIdx.x & 63) == 0 && part != 0.0)
+        atomicAdd(loss, part / ((double)B * n_out));
+}
+
+// db[n] += sum_b dZ[b][n]
+__global__ void k_colsum(int B, int N, const bf16 *__restrict__ dZ, float *__restrict__ db)
+{
+    int n = blockIdx.y * blockDim.x + threadIdx.x;
+    if (n >= N)
+        return;
+    int rows = (B + gridDim.x - 1) / gridDim.x;
+    int b0 = blockIdx.x * rows, b1 = min(B, b0 + rows);
+    float s = 0.f;
+    for (int b = b0; b < b1; b++)
+        s += bf2f(dZ[(long long)b * N + n]);
+    atomicAdd(&db[n], s);
+}
+
+// Adam on fp32 master parameters; refreshes the bf16 copy
+__global__ void k_adam(long long n, float *__restrict__ p, float *__restrict__ g, float *__restrict__ m,
+                       float *__restrict__ v, bf16 *__restrict__ pb, float lr, float b1, float b2, float eps,
+                       float c1, float c2)
+{
+    long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    float gi = g[i];
+    float mi = b1 * m[i] + (1.f - b1) * gi;
+    float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    float pi = p[i] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+    p[i] = pi;
+    if (pb)
+        pb[i] = f2bf(pi);
+    g[i] = 0.f;
+}
+
+__global__ void k_to_bf16(long long n, const float *__restrict__ a, bf16 *__restrict__ b)
+{
+    long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (i < n)
+        b[i] = f2bf(a[i]);
+}
+
+unsigned int nblk(long long n, int b = 256) { return (unsigned int)((n + b - 1) / b); }
+
+} // namespace
+
+struct eu3d_mlp {
+    int n_in, n_out, H, max_batch;
+    int K[kLayers], N[kLayers]; // layer l: K[l] -> N[l]
+    hipStream_t stream;
+    float *W[kLayers], *b[kLayers], *gW[kLayers], *gb[kLayers], *mW[kLayers], *vW[kLayers], *mb[kLayers],
+        *vb[kLayers];
+    bf16 *Wb[kLayers];
+    float *norm; // in mean[kIn], in std[kIn], out mean[kOut], out std[kOut]
+    bf16 *X0, *Hb[kLayers - 1], *dZ[kLayers];
+    float *Z[kLayers - 1], *Y, *T0;
+    double *loss;
+    float *stage;      // [max_batch x (n_in + n_out)] host-batch staging
+    long long *perm;   // [ds_n] device copy of the epoch's shuffle
+    float *ds_x, *ds_t;
+    long long ds_n;
+    long long adam_t;
+    std::string err;
+};
+
+#define MCK(call)                                                                                 \
+    do {                                                                                          \
+        hipError_t e_ = (call);                                                                   \
+        if (e_ != hipSuccess) {                                                                   \
+            mlp->err = std::string(#call) + ": " + hipGetErrorString(e_);                         \
+            return EU3D_ERR_HIP;                                                                  \
+        }                                                                                         \
+    } while (0)
+
+static int mlp_forward(eu3d_mlp *mlp, hipStream_t st, int B, bool keep_z)
+{
+    const int H = mlp->H;
+    dim3 blk(256), grid_h(nblk(B, 64), nblk(H, 64), 1);
+    for (int l = 0; l < kLayers - 1; l++) {
+        const bf16 *in = l == 0 ? mlp->X0 : mlp->Hb[l - 1];
+        const int K = mlp->K[l];
+        if (keep_z)
+            k_gemm<EPI_BIAS_GELU><<<grid_h, blk, 0, st>>>(B, H, K, in, K, 1, mlp->Wb[l], H, 1, mlp->b[l], mlp->Z[l],
+                                                         mlp->Hb[l], H, nullptr);
+        else
+            k_gemm<EPI_GELU_ONLY><<<grid_h, blk, 0, st>>>(B, H, K, in, K, 1, mlp->Wb[l], H, 1, mlp->b[l], nullptr,
+                                                         mlp->Hb[l], H, nullptr);
+    }
+    k_gemm<EPI_BIAS><<<dim3(nblk(B, 64), 1, 1), blk, 0, st>>>(B, kOut, H, mlp->Hb[kLayers - 2], H, 1,
+                                                             mlp->Wb[kLayers - 1], kOut, 1, mlp->b[kLayers - 1],
+                                                             mlp->Y, nullptr, kOut, nullptr);
+    MCK(hipGetLastError());
+    return EU3D_OK;
+}
+
+static int mlp_backward(eu3d_mlp *mlp, int B)
+{
+    dim3 blk(256);
+    const bf16 *Ain[kLayers] = {mlp->X0, mlp->Hb[0], mlp->Hb[1], mlp->Hb[2]};
+    for (int l = kLayers - 1; l >= 0; l--) {
+        const int K = mlp->K[l], N = mlp->N[l];
+        // dW[K x N] += Ain^T . dZ_l, split over the batch
+        int split = 1;
+        while (split < 64 && (B / (split * 2)) % 16 == 0 && B / (split * 2) >= 1024)
+            split *= 2;
+        k_gemm<EPI_ATOMIC><<<dim3(nblk(K, 64), nblk(N, 64), split), blk, 0, mlp->stream>>>(
+            K, N, B, Ain[l], 1, K, mlp->dZ[l], N, 1, nullptr, mlp->gW[l], nullptr, N, nullptr);
+        k_colsum<<<dim3(64, nblk(N, 64)), 64, 0, mlp->stream>>>(B, N, mlp->dZ[l], mlp->gb[l]);
+        if (l > 0) // dZ_{l-1} = (dZ_l . W_l^T) * gelu'(Z_{l-1})
+            k_gemm<EPI_GELU_BWD><<<dim3(nblk(B, 64), nblk(K, 64), 1), blk, 0, mlp->stream>>>(
+                B, K, N, mlp->dZ[l], N, 1, mlp->Wb[l], 1, N, nullptr, nullptr, mlp->dZ[l - 1], K, mlp->Z[l - 1]);
+    }
+    MCK(hipGetLastError());
+    return EU3D_OK;
+}
+
+extern "C" {
+
+const char *eu3d_mlp_last_error(const eu3d_mlp *m) { return m ? m->err.c_str() : "null network"; }
+
+int eu3d_mlp_destroy(eu3d_mlp *mlp)
+{
+    if (!mlp)
+        return EU3D_OK;
+    if (mlp->stream)
+        (void)hipStreamSynchronize(mlp->stream);
+    for (int l = 0; l < kLayers; l++) {
+        void *p[] = {mlp->W[l], mlp->b[l], mlp->gW[l], mlp->gb[l], mlp->mW[l], mlp->vW[l], mlp->mb[l], mlp->vb[l],
+                     mlp->Wb[l], mlp->dZ[l]};
+        for (void *q : p)
+            if (q)
+                (void)hipFree(q);
+    }
+    for (int l = 0; l < kLayers - 1; l++) {
+        if (mlp->Hb[l])
+            (void)hipFree(mlp->Hb[l]);
+        if (mlp->Z[l])
+            (void)hipFree(mlp->Z[l]);
+    }
+    void *p[] = {mlp->norm, mlp->X0, mlp->Y, mlp->T0, mlp->loss, mlp->stage, mlp->perm, mlp->ds_x, mlp->ds_t};
+    for (void *q : p)
+        if (q)
+            (void)hipFree(q);
+    if (mlp->stream)
+        (void)hipStreamDestroy(mlp->stream);
+    delete mlp;
+    return EU3D_OK;
+}
+
+int eu3d_mlp_create(int n_in, int hidden, int n_out, int max_batch, unsigned long long seed, eu3d_mlp **out)
+{
+    if (!out)
+        return EU3D_ERR_ARG;
+    *out = nullptr;
+    if (n_in < 1 || n_in > kIn || n_out < 1 || n_out > kOut || hidden < 64 || hidden % 64 || max_batch < 64 ||
+        max_batch % 64)
+        return EU3D_ERR_ARG;
+    eu3d_mlp *mlp = new eu3d_mlp();
+    *out = mlp;
+    mlp->n_in = n_in;
+    mlp->n_out = n_out;
+    mlp->H = hidden;
+    mlp->max_batch = max_batch;
+    const int H = hidden;
+    int K[kLayers] = {kIn, H, H, H}, N[kLayers] = {H, H, H, kOut};
+    MCK(hipStreamCreate(&mlp->stream));
+    uint64_t st = seed * 0x9E3779B97F4A7C15ull + 1;
+    auto rnd = [&]() { // splitmix64 -> uniform [-1, 1)
+        st += 0x9E3779B97F4A7C15ull;
+        uint64_t z = st;
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        z ^= z >> 31;
+        return (double)(z >> 11) * (2.0 / 9007199254740992.0) - 1.0;
+    };
+    for (int l = 0; l < kLayers; l++) {
+        mlp->K[l] = K[l];
+        mlp->N[l] = N[l];
+        size_t nw = (size_t)K[l] * N[l];
+        std::vector<float> w(nw, 0.f), bz(N[l], 0.f);
+        int kin = l == 0 ? n_in : K[l], nout = l == kLayers - 1 ? n_out : N[l];
+        double a = std::sqrt(6.0 / (kin + nout)); // Glorot uniform on the real block
+        for (int k = 0; k < kin; k++)
+            for (int n = 0; n < nout; n++)
+                w[(size_t)k * N[l] + n] = (float)(a * rnd());
+        MCK(hipMalloc(&mlp->W[l], nw * 4));
+        MCK(hipMalloc(&mlp->gW[l], nw * 4));
+        MCK(hipMalloc(&mlp->mW[l], nw * 4));
+        MCK(hipMalloc(&mlp->vW[l], nw * 4));
+        MCK(hipMalloc(&mlp->Wb[l], nw * 2));
+        MCK(hipMalloc(&mlp->b[l], N[l] * 4));
+        MCK(hipMalloc(&mlp->gb[l], N[l] * 4));
+        MCK(hipMalloc(&mlp->mb[l], N[l] * 4));
+        MCK(hipMalloc(&mlp->vb[l], N[l] * 4));
+        MCK(hipMemcpy(mlp->W[l], w.data(), nw * 4, hipMemcpyHostToDevice));
+        MCK(hipMemcpy(mlp->b[l], bz.data(), N[l] * 4, hipMemcpyHostToDevice));
+        MCK(hipMemset(mlp->gW[l], 0, nw * 4));
+        MCK(hipMemset(mlp->mW[l], 0, nw * 4));
+        MCK(hipMemset(mlp->vW[l], 0, nw * 4));
+        MCK(hipMemset(mlp->gb[l], 0, N[l] * 4));
+        MCK(hipMemset(mlp->mb[l], 0, N[l] * 4));
+        MCK(hipMemset(mlp->vb[l], 0, N[l] * 4));
+        k_to_bf16<<<nblk(nw), 256, 0, mlp->stream>>>(nw, mlp->W[l], mlp->Wb[l]);
+        MCK(hipMalloc(&mlp->dZ[l], (size_t)max_batch * N[l] * 2));
+    }
+    for (int l = 0; l < kLayers - 1; l++) {
+        MCK(hipMalloc(&mlp->Hb[l], (size_t)max_batch * H * 2));
+        MCK(hipMalloc(&mlp->Z[l], (size_t)max_batch * H * 4));
+    }
+    MCK(hipMalloc(&mlp->X0, (size_t)max_batch * kIn * 2));
+    MCK(hipMalloc(&mlp->Y, (size_t)max_batch * kOut * 4));
+    MCK(hipMalloc(&mlp->T0, (size_t)max_batch * kOut * 4));
+    MCK(hipMalloc(&mlp->loss, sizeof(double)));
+    MCK(hipMalloc(&mlp->stage, (size_t)max_batch * (n_in + n_out) * 4));
+    MCK(hipMalloc(&mlp->norm, (2 * kIn + 2 * kOut) * 4));
+    std::vector<float> nrm(2 * kIn + 2 * kOut, 0.f);
+    for (int c = 0; c < kIn; c++)
+        nrm[kIn + c] = 1.f;
+    for (int c = 0; c < kOut; c++)
+        nrm[2 * kIn + kOut + c] = 1.f;
+    MCK(hipMemcpy(mlp->norm, nrm.data(), nrm.size() * 4, hipMemcpyHostToDevice));
+    MCK(hipStreamSynchronize(mlp->stream));
+    return EU3D_OK;
+}
+
+int eu3d_mlp_set_norm(eu3d_mlp *mlp, const float *in_mean, const float *in_std, const float *out_mean,
+                      const float *out_std)
+{
+    if (!mlp || !in_mean || !in_std || !out_mean || !out_std)
+        return EU3D_ERR_ARG;
+    std::vector<float> nrm(2 * kIn + 2 * kOut, 0.f);
+    for (int c = 0; c < kIn; c++) {
+        nrm[c] = c < mlp->n_in ? in_mean[c] : 0.f;
+        nrm[kIn + c] = c < mlp->n_in ? in_std[c] : 1.f;
+    }
+    for (int c = 0; c < kOut; c++) {
+        nrm[2 * kIn + c] = c < mlp->n_out ? out_mean[c] : 0.f;
+        nrm[2 * kIn + kOut + c] = c < mlp->n_out ? out_std[c] : 1.f;
+    }
+    MCK(hipMemcpy(mlp->norm, nrm.data(), nrm.size() * 4, hipMemcpyHostToDevice));
+    return EU3D_OK;
+}
+
+int eu3d_mlp_get_norm(eu3d_mlp *mlp, float *in_mean, float *in_std, float *out_mean, float *out_std)
+{
+    if (!mlp || !in_mean || !in_std || !out_mean || !out_std)
+        return EU3D_ERR_ARG;
+    std::vector<float> nrm(2 * kIn + 2 * kOut);
+    MCK(hipMemcpy(nrm.data(), mlp->norm, nrm.size() * 4, hipMemcpyDeviceToHost));
+    for (int c = 0; c < mlp->n_in; c++) {
+        in_mean[c] = nrm[c];
+        in_std[c] = nrm[kIn + c];
+    }
+    for (int c = 0; c < mlp->n_out; c++) {
+        out_mean[c] = nrm[2 * kIn + c];
+        out_std[c] = nrm[2 * kIn + kOut + c];
+    }
+    return EU3D_OK;
+}
+
+int eu3d_mlp_shape(const eu3d_mlp *mlp, int *n_in, int *hidden, int *n_out, int *max_batch)
+{
+    if (!mlp)
+        return EU3D_ERR_ARG;
+    if (n_in)
+        *n_in = mlp->n_in;
+    if (hidden)
+        *hidden = mlp->H;
+    if (n_out)
+        *n_out = mlp->n_out;
+    if (max_batch)
+        *max_batch = mlp->max_batch;
+    return EU3D_OK;
+}
+
+// Parameters as fp32, layer by layer: W0 [kIn x H], b0 [H], ..., W3 [H x kOut], b3 [kOut].
+long long eu3d_mlp_param_count(const eu3d_mlp *mlp)
+{
+    if (!mlp)
+        return -1;
+    long long n = 0;
+    for (int l = 0; l < kLayers; l++)
+        n += (long long)mlp->K[l] * mlp->N[l] + mlp->N[l];
+    return n;
+}
+
+static int params_io(eu3d_mlp *mlp, float *host, int to_host, int which /*0 params, 1 grads*/)
+{
+    long long o = 0;
+    for (int l = 0; l < kLayers; l++) {
+        size_t nw = (size_t)mlp->K[l] * mlp->N[l];
+        float *W = which ? mlp->gW[l] : mlp->W[l], *b = which ? mlp->gb[l] : mlp->b[l];
+        if (to_host) {
+            MCK(hipMemcpy(host + o, W, nw * 4, hipMemcpyDeviceToHost));
+            MCK(hipMemcpy(host + o + nw, b, mlp->N[l] * 4, hipMemcpyDeviceToHost));
+        } else {
+            MCK(hipMemcpy(W, host + o, nw * 4, hipMemcpyHostToDevice));
+            MCK(hipMemcpy(b, host + o + nw, mlp->N[l] * 4, hipMemcpyHostToDevice));
+            k_to_bf16<<<nblk(nw), 256, 0, mlp->stream>>>(nw, W, mlp->Wb[l]);
+        }
+        o += nw + mlp->N[l];
+    }
+    MCK(hipStreamSynchronize(mlp->stream));
+    return EU3D_OK;
+}
+
+int eu3d_mlp_get_params(eu3d_mlp *mlp, float *host)
+{
+    if (!mlp || !host)
+        return EU3D_ERR_ARG;
+    MCK(hipStreamSynchronize(mlp->stream));
+    return params_io(mlp, host, 1, 0);
+}
+
+int eu3d_mlp_set_params(eu3d_mlp *mlp, const float *host)
+{
+    if (!mlp || !host)
+        return EU3D_ERR_ARG;
+    return params_io(mlp, const_cast<float *>(host), 0, 0);
+}
+
+// Loss and parameter gradients of one batch given on the host (raw features
+// [B x n_in], raw targets [B x n_out]); gradients are returned in the
+// eu3d_mlp_get_params layout.  No update.
+int eu3d_mlp_gradients(eu3d_mlp *mlp, const float *x, const float *t, int B, float *grads, double *loss)
+{
+    if (!mlp || !x || !t || !grads || B < 64 || B % 64 || B > mlp->max_batch)
+        return EU3D_ERR_ARG;
+    float *dx = mlp->stage, *dt = mlp->stage + (size_t)B * mlp->n_in;
+    MCK(hipMemcpy(dx, x, (size_t)B * mlp->n_in * 4, hipMemcpyHostToDevice));
+    MCK(hipMemcpy(dt, t, (size_t)B * mlp->n_out * 4, hipMemcpyHostToDevice));
+    MCK(hipMemsetAsync(mlp->loss, 0, sizeof(double), mlp->stream));
+    for (int l = 0; l < kLayers; l++) {
+        MCK(hipMemsetAsync(mlp->gW[l], 0, (size_t)mlp->K[l] * mlp->N[l] * 4, mlp->stream));
+        MCK(hipMemsetAsync(mlp->gb[l], 0, (size_t)mlp->N[l] * 4, mlp->stream));
+    }
+    k_prep<<<nblk(B), 256, 0, mlp->stream>>>(B, mlp->n_in, mlp->n_out, nullptr, dx, dt, mlp->norm, mlp->X0,
+                                             mlp->T0);
+    int rc = mlp_forward(mlp, mlp->stream, B, true);
+    if (rc)
+        return rc;
+    k_loss<<<nblk(B), 256, 0, mlp->stream>>>(B, mlp->n_out, mlp->Y, mlp->T0, mlp->dZ[kLayers - 1], mlp->loss);
+    if ((rc = mlp_backward(mlp, B)))
+        return rc;
+    MCK(hipStreamSynchronize(mlp->stream));
+    if (loss)
+        MCK(hipMemcpy(loss, mlp->loss, sizeof(double), hipMemcpyDeviceToHost));
+    if ((rc = params_io(mlp, grads, 1, 1)))
+        return rc;
+    for (int l = 0; l < kLayers; l++) { // training expects zeroed accumulators
+        MCK(hipMemset(mlp->gW[l], 0, (size_t)mlp->K[l] * mlp->N[l] * 4));
+        MCK(hipMemset(mlp->gb[l], 0, (size_t)mlp->N[l] * 4));
+    }
+    return EU3D_OK;
+}
+
+// Forward of B rows of raw features on the host; outputs de-normalized [B x n_out].
+int eu3d_mlp_predict(eu3d_mlp *mlp, const float *x, long long B, float *y)
+{
+    if (!mlp || !x || !y || B < 1)
+        return EU3D_ERR_ARG;
+    std::vector<float> out((size_t)mlp->max_batch * kOut), nrm(2 * kIn + 2 * kOut);
+    MCK(hipMemcpy(nrm.data(), mlp->norm, nrm.size() * 4, hipMemcpyDeviceToHost));
+    float *dx = mlp->stage;
+    for (long long b0 = 0; b0 < B; b0 += mlp->max_batch) {
+        int n = (int)std::min<long long>(mlp->max_batch, B - b0);
+        int np = (n + 63) / 64 * 64;
+        MCK(hipMemsetAsync(dx, 0, (size_t)np * mlp->n_in * 4, mlp->stream));
+        MCK(hipMemcpyAsync(dx, x + (size_t)b0 * mlp->n_in, (size_t)n * mlp->n_in * 4, hipMemcpyHostToDevice,
+                           mlp->stream));
+        k_prep<<<nblk(np), 256, 0, mlp->stream>>>(np, mlp->n_in, mlp->n_out, nullptr, dx, nullptr, mlp->norm,
+                                                  mlp->X0, nullptr);
+        int rc = mlp_forward(mlp, mlp->stream, np, false);
+        if (rc)
+            return rc;
+        MCK(hipMemcpyAsync(out.data(), mlp->Y, (size_t)np * kOut * 4, hipMemcpyDeviceToHost, mlp->stream));
+        MCK(hipStreamSynchronize(mlp->stream));
+        for (int i = 0; i < n; i++)
+            for (int c = 0; c < mlp->n_out; c++)
+                y[(size_t)(b0 + i) * mlp->n_out + c] =
+                    out[(size_t)i * kOut + c] * nrm[2 * kIn + kOut + c] + nrm[2 * kIn + c];
+    }
+    return EU3D_OK;
+}
+
+// Device-resident training set (raw features and targets).
+int eu3d_mlp_set_dataset(eu3d_mlp *mlp, const float *x, const float *t, long long n)
+{
+    if (!mlp || !x || !t || n < 64)
+        return EU3D_ERR_ARG;
+    if (mlp->perm)
+        MCK(hipFree(mlp->perm));
+    mlp->perm = nullptr;
+    if (mlp->ds_x)
+        MCK(hipFree(mlp->ds_x));
+    if (mlp->ds_t)
+        MCK(hipFree(mlp->ds_t));
+    MCK(hipMalloc(&mlp->ds_x, (size_t)n * mlp->n_in * 4));
+    MCK(hipMalloc(&mlp->ds_t, (size_t)n * mlp->n_out * 4));
+    MCK(hipMemcpy(mlp->ds_x, x, (size_t)n * mlp->n_in * 4, hipMemcpyHostToDevice));
+    MCK(hipMemcpy(mlp->ds_t, t, (size_t)n * mlp->n_out * 4, hipMemcpyHostToDevice));
+    MCK(hipMalloc(&mlp->perm, (size_t)n * sizeof(long long)));
+    mlp->ds_n = n;
+    return EU3D_OK;
+}
+
+// Adam epochs over the device-resident set with batch B (shuffled per epoch
+// on the host with a seeded generator); the learning rate follows a cosine
+// from lr to lr_end over the call's steps.  Mean loss per epoch -> losses.
+int eu3d_mlp_train(eu3d_mlp *mlp, int epochs, int B, float lr, float lr_end, unsigned long long seed,
+                   double *losses)
+{
+    if (!mlp || !mlp->ds_x || B < 64 || B % 64 || B > mlp->max_batch || epochs < 1 || B > mlp->ds_n)
+        return EU3D_ERR_ARG;
+    const long long n = mlp->ds_n, steps = n / B;
+    std::vector<long long> perm(n);
+    for (long long i = 0; i < n; i++)
+        perm[i] = i;
+    uint64_t st = seed * 0x2545F4914F6CDD1Dull + 7;
+    auto next = [&]() {
+        st += 0x9E3779B97F4A7C15ull;
+        uint64_t z = st;
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        return z ^ (z >> 31);
+    };
+    long long *dperm = mlp->perm;
+    const long long total = steps * epochs;
+    const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+    for (int e = 0; e < epochs; e++) {
+        for (long long i = n - 1; i > 0; i--) {
+            long long j = (long long)(next() % (uint64_t)(i + 1));
+            std::swap(perm[i], perm[j]);
+        }
+        MCK(hipMemcpyAsync(dperm, perm.data(), (size_t)n * sizeof(long long), hipMemcpyHostToDevice, mlp->stream));
+        MCK(hipMemsetAsync(mlp->loss, 0, sizeof(double), mlp->stream));
+        for (long long s = 0; s < steps; s++) {
+            k_prep<<<nblk(B), 256, 0, mlp->stream>>>(B, mlp->n_in, mlp->n_out, dperm + s * B, mlp->ds_x, mlp->ds_t,
+                                                     mlp->norm, mlp->X0, mlp->T0);
+            int rc = mlp_forward(mlp, mlp->stream, B, true);
+            if (rc)
+                return rc;
+            k_loss<<<nblk(B), 256, 0, mlp->stream>>>(B, mlp->n_out, mlp->Y, mlp->T0, mlp->dZ[kLayers - 1],
+                                                     mlp->loss);
+            if ((rc = mlp_backward(mlp, B)))
+                return rc;
+            mlp->adam_t += 1;
+            const double frac = (double)(e * steps + s) / (double)total;
+            const float lr_s = (float)(lr_end + 0.5 * (lr - lr_end) * (1.0 + std::cos(3.14159265358979 * frac)));
+            float c1 = 1.f - std::pow(b1, (float)mlp->adam_t), c2 = 1.f - std::pow(b2, (float)mlp->adam_t);
+            for (int l = 0; l < kLayers; l++) {
+                long long nw = (long long)mlp->K[l] * mlp->N[l];
+                k_adam<<<nblk(nw), 256, 0, mlp->stream>>>(nw, mlp->W[l], mlp->gW[l], mlp->mW[l], mlp->vW[l],
+                                                          mlp->Wb[l], lr_s, b1, b2, eps, c1, c2);
+                k_adam<<<nblk(mlp->N[l]), 256, 0, mlp->stream>>>(mlp->N[l], mlp->b[l], mlp->gb[l], mlp->mb[l],
+                                                                 mlp->vb[l], nullptr, lr_s, b1, b2, eps, c1, c2);
+            }
+        }
+        double L = 0.0;
+        MCK(hipMemcpyAsync(&L, mlp->loss, sizeof(double), hipMemcpyDeviceToHost, mlp->stream));
+        MCK(hipStreamSynchronize(mlp->stream));
+        if (losses)
+            losses[e] = L / (double)steps;
+    }
+    return EU3D_OK;
+}
+
+} // extern "C"
+
+namespace eu3d_mlp_internal {
+
+View view(eu3d_mlp *mlp) { return View{mlp->X0, mlp->Y, mlp->norm, mlp->n_in, mlp->n_out, mlp->max_batch}; }
+
+int forward(eu3d_mlp *mlp, hipStream_t stream, int B)
+{
+    if (B < 64 || B % 64 || B > mlp->max_batch)
+        return EU3D_ERR_ARG;
+    return mlp_forward(mlp, stream, B, false);
+}
+
+} // namespace eu3d_mlp_internal

@@ -1,0 +1,124 @@
+/*
+ * eu3d_surrogate.h -- C ABI of the learned chemistry surrogate.
+ *
+ * No reference counterpart: the reference declares a DNN chemistry mode
+ * (include/Global.hpp:16-21, ctrl keyword parsing src/Euler.cpp:150-156) whose
+ * step, Euler::DNN (src/Euler.cpp:1473-1480), runs no chemistry at all.  This
+ * header fills that slot.  The FP64 adaptive trapezoid (eu3d_chemistry) stays
+ * the ground truth: the network replaces it only in cells whose predicted
+ * sub-step count is large, and every such cell falls back to the FP64
+ * integrator when a check fails.
+ *
+ * Network: MLP n_in -> H -> H -> H -> n_out with GELU, BF16 weights and
+ * activations, FP32 accumulation on MFMA, FP32 master weights, Adam.
+ * Features (n_in = ns + 3): T [K], ln p [Pa], ln(Y_s + 1e-12) for each
+ * species, ln dt [s].  Targets (n_out = ns + 1): Y_s(t+dt) - Y_s(t), and
+ * T(t+dt) - T(t) [K].  Features and targets are normalized on the device
+ * with the table set by eu3d_mlp_set_norm.
+ *
+ * Same conventions as eu3d.h: int status, EU3D_ERR_* codes, no exits.
+ */
+#ifndef EU3D_SURROGATE_H
+#define EU3D_SURROGATE_H
+
+#include "eu3d.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct eu3d_mlp eu3d_mlp;
+
+#define EU3D_CHEM_SURROGATE 0x8 /* eu3d_chemistry flag: route stiff cells to the attached network */
+
+/* ---- the network ------------------------------------------------------- */
+/* n_in <= 32, n_out <= 32, hidden a multiple of 64, max_batch a multiple of
+ * 64 (rows per device pass).  Glorot-uniform weights from seed, zero biases. */
+int eu3d_mlp_create(int n_in, int hidden, int n_out, int max_batch, unsigned long long seed, eu3d_mlp **out);
+int eu3d_mlp_destroy(eu3d_mlp *mlp);
+const char *eu3d_mlp_last_error(const eu3d_mlp *mlp);
+int eu3d_mlp_shape(const eu3d_mlp *mlp, int *n_in, int *hidden, int *n_out, int *max_batch);
+
+/* Normalization: x' = (x - in_mean) / in_std, t' = (t - out_mean) / out_std. */
+int eu3d_mlp_set_norm(eu3d_mlp *mlp, const float *in_mean, const float *in_std, const float *out_mean,
+                      const float *out_std);
+int eu3d_mlp_get_norm(eu3d_mlp *mlp, float *in_mean, float *in_std, float *out_mean, float *out_std);
+
+/* FP32 parameters, layer by layer: W0 [32 x H] (row = padded input), b0 [H],
+ * W1 [H x H], b1, W2 [H x H], b2, W3 [H x 32], b3 [32]; padded rows/columns
+ * must be 0.  Setting them also refreshes the BF16 copies. */
+long long eu3d_mlp_param_count(const eu3d_mlp *mlp);
+int eu3d_mlp_get_params(eu3d_mlp *mlp, float *params);
+int eu3d_mlp_set_params(eu3d_mlp *mlp, const float *params);
+
+/* Mean-squared error (over the n_out real outputs, normalized units) of one
+ * batch of B rows (B a multiple of 64, <= max_batch) and its gradients with
+ * respect to the FP32 parameters, in the eu3d_mlp_get_params layout.  The
+ * parameters are not changed. */
+int eu3d_mlp_gradients(eu3d_mlp *mlp, const float *x, const float *t, int B, float *grads, double *loss);
+
+/* Forward of B rows of raw features; de-normalized outputs y[B x n_out]. */
+int eu3d_mlp_predict(eu3d_mlp *mlp, const float *x, long long B, float *y);
+
+/* Training: the set is copied to the device once; each epoch is a seeded
+ * shuffle and n / B Adam steps (beta 0.9 / 0.999, eps 1e-8) with a cosine
+ * learning rate from lr to lr_end over the call.  losses[epochs]: mean batch
+ * loss of each epoch (may be NULL). */
+int eu3d_mlp_set_dataset(eu3d_mlp *mlp, const float *x, const float *t, long long n);
+int eu3d_mlp_train(eu3d_mlp *mlp, int epochs, int B, float lr, float lr_end, unsigned long long seed,
+                   double *losses);
+
+/* ---- the network inside the chemistry ---------------------------------- */
+/* Acceptance checks of a network update (all must hold, else the cell is
+ * integrated by the FP64 trapezoid from its unchanged state):
+ *  - every feature lies in [hull_lo, hull_hi] (the training set's box);
+ *  - the element-conservation projection (dY -> proj . dY, proj = I - A^T
+ *    (A A^T)^-1 A for the element mass-fraction matrix A) moves dY by at
+ *    most tol_elem * (sum |proj . dY| + 1e-6) in the 1-norm;
+ *  - Y_s + dY_s >= -y_floor for every species (small negatives clip to 0);
+ *  - the temperature recovered from the new composition at the cell's
+ *    internal energy (the Newton of Update_after_CS) is finite and within
+ *    tol_T of T + dT_net. */
+typedef struct {
+    int min_nchem;      /* cells with predicted Nchem >= min_nchem (>= 1) use the network */
+    double tol_T;       /* K */
+    double tol_elem;    /* relative */
+    double y_floor;     /* absolute mass fraction */
+    float hull_lo[32], hull_hi[32]; /* raw feature box, first n_in entries used */
+    double proj[16 * 16];           /* ns x ns, row-major */
+} eu3d_surrogate_params;
+
+/* Attach (mlp != NULL) or detach (mlp == NULL) a network; the network must
+ * have n_in = ns + 3 and n_out = ns + 1 and live on the context's device.
+ * The context does not own it.  EU3D_CHEM_SURROGATE requires
+ * EU3D_CHEM_COMPACT. */
+int eu3d_set_surrogate(eu3d_ctx *ctx, eu3d_mlp *mlp, const eu3d_surrogate_params *params);
+
+/* Counters of the last chemistry call. */
+typedef struct {
+    long long routed;   /* cells sent to the network */
+    long long fallback; /* of those, integrated by the FP64 trapezoid after a failed check */
+    long long fail_hull, fail_elem, fail_neg, fail_T;
+} eu3d_surrogate_stats;
+int eu3d_surrogate_stats_get(eu3d_ctx *ctx, eu3d_surrogate_stats *out);
+
+/* ---- training data ------------------------------------------------------ */
+/* Sets the first n interior cells (interior order: i slowest, k fastest) to
+ * rest at temperature T[c], pressure p[c] and mass fractions Y[c*ns + s]
+ * (state built as in Flowfield::FieldInitial, src/Flowfield.cpp:193-273);
+ * the other cells are unchanged. */
+int eu3d_set_cells(eu3d_ctx *ctx, long long n, const double *T, const double *p, const double *Y);
+
+/* One FP64 chemistry call (flags as eu3d_chemistry, without SURROGATE) that
+ * also records, for every interior cell whose predicted Nchem >= min_nchem,
+ * its features before and its targets after the call, in interior order:
+ * x[rows x (ns+3)], t[rows x (ns+1)], at most max_rows rows; *rows = number
+ * of qualifying cells (may exceed max_rows: then only the first max_rows
+ * are written). */
+int eu3d_chemistry_harvest(eu3d_ctx *ctx, double dt, int flags, int min_nchem, long long max_rows, float *x,
+                           float *t, long long *rows);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

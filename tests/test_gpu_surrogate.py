@@ -1,0 +1,297 @@
+"""GPU tests of the chemistry surrogate: the BF16 MFMA network against its CPU
+restatement (oracle/mlp_oracle.py), device training, the training-data
+harvest against the FP64 chemistry, and the surrogate path inside the
+chemistry (routing, checks, FP64 fallback)."""
+import importlib
+import os
+
+import numpy as np
+import pytest
+
+import mlp_oracle as mo
+from eu3d_load import eu3d
+
+sur = importlib.import_module("eu3d-gpu_amd.surrogate")
+
+pytestmark = pytest.mark.gpu
+
+MECH = os.path.join(eu3d.INPUT, "H2_JA919.txt")
+NAMES = sur.species_names(MECH)
+NS = len(NAMES)
+N_IN, N_OUT = NS + 3, NS + 1
+
+
+def _net(seed, H=256, max_batch=1024):
+    """A network with Glorot weights, random biases and a random normalization,
+    plus the same parameters for the oracle."""
+    net = sur.Network(N_IN, N_OUT, hidden=H, max_batch=max_batch, seed=seed)
+    rng = np.random.default_rng(seed)
+    layers = mo.unpack(mo.init_params(H, N_IN, N_OUT, seed).astype(np.float64), H)
+    for l, (W, b) in enumerate(layers):
+        nout = N_OUT if l == 3 else b.size
+        b[:nout] = rng.normal(0, 0.1, nout)
+    p = mo.pack(layers).astype(np.float32)
+    net.set_params(p)
+    norm = (rng.normal(0, 1, 32), 0.5 + rng.random(32), rng.normal(0, 1, 32), 0.5 + rng.random(32))
+    net.set_norm(norm[0][:N_IN], norm[1][:N_IN], norm[2][:N_OUT], norm[3][:N_OUT])
+    norm = tuple(np.asarray(v, np.float32).astype(np.float64) for v in norm)
+    return net, p, norm
+
+
+def _inputs(norm, B, seed):
+    rng = np.random.default_rng(seed)
+    return (norm[0][:N_IN] + norm[1][:N_IN] * rng.normal(size=(B, N_IN))).astype(np.float32)
+
+
+def _pad_mask(H):
+    """True at the parameters of padded inputs / outputs."""
+    pad = np.ones(mo.param_count(H), bool)
+    o = 0
+    for l, (k, n) in enumerate(mo.shapes(H)):
+        kin = N_IN if l == 0 else k
+        nout = N_OUT if l == 3 else n
+        m = np.zeros((k, n), bool)
+        m[:kin, :nout] = True
+        pad[o:o + k * n] = ~m.ravel()
+        o += k * n
+        pad[o:o + n] = np.arange(n) >= nout
+        o += n
+    return pad
+
+
+def test_mlp_params_roundtrip():
+    net, p, _ = _net(1)
+    q = net.get_params()
+    assert q.size == mo.param_count(256)
+    assert np.array_equal(p, q)
+
+
+def test_mlp_predict_matches_oracle():
+    """Forward through the MFMA kernels (also pins the instruction's operand
+    and accumulator lane layout: a transposed tile gives O(1) errors).
+    B = 1000 exercises the zero-padded last tile."""
+    net, p, norm = _net(2)
+    x = _inputs(norm, 1000, 3)
+    y = net.predict(x).astype(np.float64)
+    ref = mo.predict(p, 256, x, norm, N_IN, N_OUT, emulate=True)
+    exact = mo.predict(p, 256, x, norm, N_IN, N_OUT, emulate=False)
+    scale = np.abs(ref).max()
+    err = np.abs(y - ref)
+    # same BF16 rounding points, FP32 vs FP64 accumulation
+    assert err.max() <= 2e-3 * scale, err.max() / scale
+    assert err.mean() <= 2e-4 * np.abs(ref).mean(), err.mean()
+    # BF16 as a whole stays close to the exact network
+    assert np.abs(y - exact).max() <= 3e-2 * scale
+
+
+def test_mlp_gradients_match_oracle():
+    net, p, norm = _net(4)
+    B = 512
+    x = _inputs(norm, B, 5)
+    rng = np.random.default_rng(6)
+    t = (norm[2][:N_OUT] + norm[3][:N_OUT] * rng.normal(size=(B, N_OUT))).astype(np.float32)
+    loss, g = net.gradients(x, t)
+    rloss, rg = mo.loss_and_grads(p, 256, x, t, norm, N_IN, N_OUT, emulate=True)
+    assert abs(loss - rloss) <= 1e-3 * rloss, (loss, rloss)
+    o = 0
+    for l, (k, n) in enumerate(mo.shapes(256)):
+        for part in (k * n, n):
+            a, b = g[o:o + part].astype(np.float64), rg[o:o + part]
+            rel = np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30)
+            assert rel <= 1e-2, (l, part, rel)
+            o += part
+    # padded inputs / outputs get exactly zero gradient
+    assert np.all(g[_pad_mask(256)] == 0)
+
+
+def test_mlp_training_reduces_loss():
+    rng = np.random.default_rng(8)
+    n = 1 << 15
+    x = rng.normal(size=(n, N_IN)).astype(np.float32)
+    t = np.stack([np.sin(x[:, i % N_IN]) * x[:, (i + 1) % N_IN] + 0.1 * i for i in range(N_OUT)], 1)
+    net = sur.Network(N_IN, N_OUT, max_batch=4096, seed=9)
+    net.fit_norm(x, t)
+    net.set_dataset(x, t.astype(np.float32))
+    losses = net.train(30, 4096, lr=3e-3, lr_end=1e-4, seed=10)
+    assert np.all(np.isfinite(losses)), losses
+    assert losses[-1] < 0.1 * losses[0], losses
+    # the padded block never moves
+    assert np.all(net.get_params()[_pad_mask(256)] == 0)
+    y = net.predict(x[:2048])
+    mse, var = np.mean((y - t[:2048]) ** 2, axis=0), np.var(t[:2048], axis=0)
+    assert np.all(mse < 0.2 * var), (mse / var, losses)
+
+
+# ---------------------------------------------------------------------------
+# solver side
+# ---------------------------------------------------------------------------
+@pytest.fixture
+def box(tmp_path):
+    def make(ni=16, nj=8, nk=8):
+        return eu3d.Solver(sur.write_box_case(str(tmp_path / f"b{ni}x{nj}x{nk}"), ni, nj, nk),
+                           base_dir=str(tmp_path))
+    return make
+
+
+def _states(s, seed):
+    """Seeded 0-D states for every interior cell of solver s, mild enough that
+    the FP64 trapezoid needs at most a few hundred sub-steps at dt = 3e-8 s
+    (the hottest, highest-pressure, radical-rich states need 1e5 and more)."""
+    mw = np.array(s.field("mw"), dtype=float)
+    return sur.sample_states(s.ni * s.nj * s.nk, NAMES, mw, seed, T_range=(1100.0, 2000.0), p_atm=(0.5, 5.0),
+                             radical_max=1e-4)
+
+
+DT = 3e-8
+
+
+def _interior_cells(s, a):
+    return s.interior(a).reshape(s.ni * s.nj * s.nk, *a.shape[3:])
+
+
+def test_set_cells_builds_consistent_states(box):
+    s = box()
+    T, p, Y = _states(s, 11)
+    sur.set_cells(s.ctx, T, p, Y)
+    s.sync()
+    assert np.allclose(_interior_cells(s, s.field("T")), T, rtol=0, atol=1e-6)
+    assert np.allclose(_interior_cells(s, s.field("P")), p, rtol=1e-9, atol=0)
+    assert np.allclose(_interior_cells(s, s.field("Yi")), Y, rtol=1e-13, atol=1e-18)
+    assert np.all(_interior_cells(s, s.field("U")) == 0)
+
+
+def _copy_state(s):
+    s.sync()
+    return {k: np.array(s.field(k)) for k in ("CS", "T", "P", "D", "E", "Yi", "C", "Gamma", "H")}
+
+
+def test_harvest_rows_match_fp64_chemistry(box):
+    dt = DT
+    a, b = box(), box()
+    n = a.ni * a.nj * a.nk
+    T, p, Y = _states(a, 12)
+    for s in (a, b):
+        sur.set_cells(s.ctx, T, p, Y)
+    x, t, rows = sur.harvest_call(a.ctx, NS, dt, eu3d.CHEM_COMPACT, 1, n)
+    assert rows == n and len(x) == n
+    before = _copy_state(b)
+    eu3d.check(eu3d.load().eu3d_chemistry(b.ctx, dt, eu3d.CHEM_COMPACT), b.ctx, "chem")
+    after = _copy_state(b)
+    # the harvested call is the plain FP64 chemistry
+    sa = _copy_state(a)
+    for k in after:
+        assert np.array_equal(sa[k], after[k]), k
+    fx = sur.features(_interior_cells(b, before["T"]), _interior_cells(b, before["P"]),
+                      _interior_cells(b, before["Yi"]), dt)
+    assert np.allclose(x, fx, rtol=2e-6, atol=2e-6)
+    dY = (_interior_cells(b, after["Yi"]) - _interior_cells(b, before["Yi"])).astype(np.float32)
+    dT = (_interior_cells(b, after["T"]) - _interior_cells(b, before["T"])).astype(np.float32)
+    assert np.array_equal(t[:, :NS], dY)
+    assert np.array_equal(t[:, NS], dT)
+    # min_nchem selects exactly the cells with Nchem >= min_nchem, in interior order
+    sur.set_cells(a.ctx, T, p, Y)
+    x2, t2, rows2 = sur.harvest_call(a.ctx, NS, dt, eu3d.CHEM_COMPACT, 3, n)
+    a.sync()
+    nch = a.interior(a.nchem()).reshape(-1)
+    assert rows2 == np.count_nonzero(nch >= 3) and rows2 > 0
+    assert np.array_equal(x2, x[nch >= 3])
+    assert np.array_equal(t2, t[nch >= 3])
+
+
+def _attach_rejecting(s, net, min_nchem):
+    lo = np.full(N_IN, 1.0, np.float32)
+    hi = np.full(N_IN, -1.0, np.float32)   # empty box: every cell fails the hull check
+    sur.attach(s.ctx, net, NAMES, min_nchem=min_nchem, hull=(lo, hi))
+
+
+@pytest.mark.parametrize("min_nchem", [1, 3])
+def test_surrogate_full_fallback_is_fp64_bitwise(box, min_nchem):
+    dt = DT
+    a, b = box(), box()
+    T, p, Y = _states(a, 13)
+    for s in (a, b):
+        sur.set_cells(s.ctx, T, p, Y)
+    net, _, _ = _net(14)
+    _attach_rejecting(a, net, min_nchem)
+    L = eu3d.load()
+    flags = eu3d.CHEM_COMPACT | eu3d.CHEM_KEEP_CMS
+    eu3d.check(L.eu3d_chemistry(a.ctx, dt, flags | eu3d.CHEM_SURROGATE), a.ctx, "chem sur")
+    eu3d.check(L.eu3d_chemistry(b.ctx, dt, flags), b.ctx, "chem")
+    st = sur.stats(a.ctx)
+    sa, sb = _copy_state(a), _copy_state(b)
+    for k in sa:
+        assert np.array_equal(sa[k], sb[k]), k
+    a.sync()
+    nch = a.interior(a.nchem()).reshape(-1)
+    assert st["routed"] == np.count_nonzero(nch >= min_nchem) > 0
+    assert st["fallback"] == st["routed"] == st["fail_hull"]
+    assert np.array_equal(a.nchem(), b.nchem())
+
+
+def test_surrogate_requires_network_and_compaction(box):
+    s = box()
+    L = eu3d.load()
+    assert L.eu3d_chemistry(s.ctx, 1e-7, eu3d.CHEM_COMPACT | eu3d.CHEM_SURROGATE) == eu3d.EU3D_ERR_STATE
+    net, _, _ = _net(15)
+    _attach_rejecting(s, net, 2)
+    assert L.eu3d_chemistry(s.ctx, 1e-7, eu3d.CHEM_SURROGATE) == eu3d.EU3D_ERR_ARG
+    bad = sur.Network(N_IN + 1, N_OUT, max_batch=64)
+    with pytest.raises(eu3d.EU3DError):
+        sur.attach(s.ctx, bad, NAMES)
+    sur.detach(s.ctx)
+    assert L.eu3d_chemistry(s.ctx, 1e-7, eu3d.CHEM_COMPACT | eu3d.CHEM_SURROGATE) == eu3d.EU3D_ERR_STATE
+
+
+def test_surrogate_trained_updates_conserve_and_fall_back(box):
+    """A briefly trained network inside the chemistry: accepted cells conserve
+    elements (projection) and mass, stay non-negative, and their recovered
+    temperature passed the check; rejected cells are exactly FP64."""
+    s = box(32, 16, 16)
+    n = s.ni * s.nj * s.nk
+    xs, ts = [], []
+    for seed in range(4):
+        T, p, Y = _states(s, 100 + seed)
+        sur.set_cells(s.ctx, T, p, Y)
+        for c in range(4):
+            x, t, _ = sur.harvest_call(s.ctx, NS, DT, eu3d.CHEM_COMPACT, 2, n)
+            xs.append(x)
+            ts.append(t)
+    x, t = np.concatenate(xs), np.concatenate(ts)
+    assert len(x) > 4096
+    net = sur.Network(N_IN, N_OUT, max_batch=4096, seed=16)
+    net.fit_norm(x, t)
+    net.hull_lo, net.hull_hi = x.min(0), x.max(0)
+    net.set_dataset(x, t)
+    losses = net.train(20, 1024, lr=2e-3, lr_end=1e-4, seed=17)
+    assert losses[-1] < 0.5 * losses[0]
+    # held-out states on two contexts: surrogate vs FP64
+    a, b = s, box(32, 16, 16)
+    T, p, Y = _states(a, 999)
+    for c in (a, b):
+        sur.set_cells(c.ctx, T, p, Y)
+    sur.attach(a.ctx, net, NAMES, min_nchem=2, tol_T=50.0, tol_elem=0.2, y_floor=1e-3)
+    L = eu3d.load()
+    eu3d.check(L.eu3d_chemistry(a.ctx, DT, eu3d.CHEM_COMPACT | eu3d.CHEM_SURROGATE), a.ctx, "chem sur")
+    eu3d.check(L.eu3d_chemistry(b.ctx, DT, eu3d.CHEM_COMPACT), b.ctx, "chem")
+    st = sur.stats(a.ctx)
+    assert st["routed"] > 0 and st["fallback"] < st["routed"]
+    sa, sb = _copy_state(a), _copy_state(b)
+    a.sync()
+    nch = a.interior(a.nchem()).reshape(-1)
+    Ya, Yb = _interior_cells(a, sa["Yi"]), _interior_cells(b, sb["Yi"])
+    Ta, Tb = _interior_cells(a, sa["T"]), _interior_cells(b, sb["T"])
+    same = np.all(Ya == Yb, axis=1) & (Ta == Tb)
+    # cells below the routing threshold are FP64
+    assert np.all(same[nch < 2])
+    acc = ~same
+    assert np.count_nonzero(acc) >= st["routed"] - st["fallback"] - 8
+    A, _ = sur.element_matrix(NAMES)
+    Y0 = Y
+    assert np.all(Ya[acc] >= 0)
+    assert np.allclose(Ya[acc].sum(1), 1.0, atol=1e-9)
+    # elements: only the clipping of small negatives may move them
+    assert np.abs((Ya[acc] - Y0[acc]) @ A.T).max() <= NS * 1e-3 + 1e-12
+    assert np.all(np.isfinite(Ta[acc]))
+    err_T = np.abs(Ta[acc] - Tb[acc])
+    print(f"routed {st['routed']} fallback {st['fallback']} accepted {np.count_nonzero(acc)} "
+          f"|dT| median {np.median(err_T):.2f} K max {err_T.max():.2f} K")

@@ -306,3 +306,69 @@ def test_rk3_is_three_ee_stages_in_shu_osher_form(tmp_path):
     c.advection()
     c.update_after_adv()
     assert not np.array_equal(I(c.field("CS")), I(a.field("CS")))
+
+
+# ---------------------------------------------------------------------------
+# the surrogate network's CPU restatement (oracle/mlp_oracle.py)
+# ---------------------------------------------------------------------------
+import mlp_oracle as mo  # noqa: E402
+
+
+def test_bf16_rounding_exact_cases():
+    one = 1.0
+    cases_ = [(one, one), (one + 2.0 ** -8, one),                    # tie -> even (down)
+              (one + 3 * 2.0 ** -8, one + 2.0 ** -6),                 # tie -> even (up)
+              (one + 2.0 ** -8 + 2.0 ** -20, one + 2.0 ** -7),        # above the tie
+              (-(one + 3 * 2.0 ** -8), -(one + 2.0 ** -6)), (0.0, 0.0), (2.0 ** -130, 2.0 ** -130)]
+    for x, want in cases_:
+        assert mo.bf16(np.array([x]))[0] == want, x
+    assert np.isinf(mo.bf16(np.array([np.inf]))[0])
+    assert np.isnan(mo.bf16(np.array([np.nan]))[0])
+    assert np.isinf(mo.bf16(np.array([3.4e38]))[0])  # rounds past the largest BF16
+
+
+def test_gelu_identities():
+    x = np.linspace(-6, 6, 241)
+    assert np.allclose(mo.gelu(x) - mo.gelu(-x), x, rtol=0, atol=1e-14)   # Phi(x) + Phi(-x) = 1
+    assert mo.gelu(np.array([0.0]))[0] == 0.0
+    h = 1e-6
+    fd = (mo.gelu(x + h) - mo.gelu(x - h)) / (2 * h)
+    assert np.allclose(mo.gelu_grad(x), fd, rtol=0, atol=1e-8)
+
+
+def test_mlp_oracle_gradients_match_central_differences():
+    H, n_in, n_out, B = 64, 12, 10, 16
+    rng = np.random.default_rng(5)
+    p = mo.init_params(H, n_in, n_out, seed=3).astype(np.float64)
+    layers = mo.unpack(p, H)
+    # nonzero biases on the real block so that every term of the chain is exercised
+    for l, (W, b) in enumerate(layers):
+        nout = n_out if l == 3 else b.size
+        b[:nout] = rng.normal(0, 0.3, nout)
+    p = mo.pack(layers)
+    x = rng.normal(size=(B, n_in))
+    t = rng.normal(size=(B, n_out))
+    norm = (rng.normal(size=32) * 0.1, 1 + rng.random(32), rng.normal(size=32) * 0.1, 1 + rng.random(32))
+    loss, g = mo.loss_and_grads(p, H, x, t, norm, n_in, n_out, emulate=False)
+    assert loss > 0
+    # real (non-padded) parameter indices of every layer
+    real, o = [], 0
+    for l, (k, n) in enumerate(mo.shapes(H)):
+        kin = n_in if l == 0 else k
+        nout = n_out if l == 3 else n
+        real += [o + a * n + c for a in range(kin) for c in range(nout)]
+        o += k * n
+        real += [o + c for c in range(nout)]
+        o += n
+    real = np.array(real)
+    pad = np.setdiff1d(np.arange(p.size), real)
+    assert np.all(g[pad] == 0.0)   # padded weights never move
+    pick = rng.choice(real, size=60, replace=False)
+    h = 1e-6
+    for i in pick:
+        pp, pm = p.copy(), p.copy()
+        pp[i] += h
+        pm[i] -= h
+        fd = (mo.loss_and_grads(pp, H, x, t, norm, n_in, n_out, emulate=False)[0] -
+              mo.loss_and_grads(pm, H, x, t, norm, n_in, n_out, emulate=False)[0]) / (2 * h)
+        assert abs(g[i] - fd) <= 1e-6 * max(1.0, abs(fd)) + 1e-9, (i, g[i], fd)
