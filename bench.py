@@ -185,7 +185,8 @@ def main():
         "metric": METRIC, "value": value, "unit": "cell-steps/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": el / args.steps * 1e3, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-        "config": {"workload": "3D reactive shock-bubble, plain FP64 path (configs[1])",
+        "config": {"workload": f"3-D reactive shock-bubble {s.ni}x{s.nj}x{s.nk}, H2-O2 bubble in N2, Mach 2, "
+                               "plain FP64 path",
                    "grid": [s.ni, s.nj, s.nk], "species": s.ns, "reactions": s.nr,
                    "scheme": "SSP-RK3 + MUSCL(minmod) + AUSM+", "chemistry": "adaptive trapezoid, compacted",
                    "cfl": 0.5, "sim_time_end": info["time"],

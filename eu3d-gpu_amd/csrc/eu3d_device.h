@@ -239,7 +239,13 @@ __device__ __forceinline__ void recover(const MT &m, const double (&cs)[NS + 4],
 __device__ __forceinline__ double lim_s(double A1, double Aden)
 {
     const double kk = 1.0 / 3.0;
-    double r = A1 / (Aden + 1e-12);
+    const double den = Aden + 1e-12;
+    // Flat stencil: A1 = +-0 over a finite or infinite non-zero den gives
+    // r = +-0 and s = +0 exactly; skip the division (a wave whose lanes all
+    // take this branch issues none of it).
+    if (__builtin_expect(A1 == 0.0 && den != 0.0 && den == den, 0))
+        return 0.0;
+    double r = A1 / den;
     double s;
     if (r > 0.0) {
         if (r < 1.0)

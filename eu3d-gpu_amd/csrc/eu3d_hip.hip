@@ -320,7 +320,8 @@ __global__ void __launch_bounds__(kBlock) k_ghost_z(Grid g, int nprim, double *p
 // stages (1 also saves CS into CS0).
 //
 // A block owns kTJ rows (j) of kTK = 64 cells (k, one wave per row) and
-// marches over kMarch planes (i).  Every face flux is computed once:
+// marches over kMarch planes (i); it has kTJ + 1 waves.  Every face flux is
+// computed once:
 //   x faces: by the owning thread, the plus face of plane i is kept in
 //            registers as the minus face of plane i+1;
 //   y faces: kTJ+1 rows of 64 faces, one per thread (the extra wave computes
@@ -329,13 +330,51 @@ __global__ void __launch_bounds__(kBlock) k_ghost_z(Grid g, int nprim, double *p
 // RHS = -dt/dx (F+ - F-) - dt/dy (G+ - G-) - dt/dz (Q+ - Q-) in that order
 // (TimeAdv::EE, src/TimeAdv.cpp:51-54).
 // ---------------------------------------------------------------------------
-constexpr int kTJ = 4;
-constexpr int kTK = 64;
-constexpr int kFluxThreads = 64 * (kTJ + 1);
-constexpr int kMarch = 16;
+// Tile shape: 7 rows + 1 halo wave = 8 waves, which fill the 8 wave slots of
+// a CU at the kernel's register occupancy (2 waves per SIMD); 8 planes per
+// block give about 5 rounds of blocks over the chip at 256 x 128 x 128.
+#ifndef EU3D_FLUX_TJ
+#define EU3D_FLUX_TJ 7
+#endif
+#ifndef EU3D_FLUX_MARCH
+#define EU3D_FLUX_MARCH 8
+#endif
 #ifndef EU3D_FLUX_WAVES_PER_EU
 #define EU3D_FLUX_WAVES_PER_EU 2
 #endif
+constexpr int kTJ = EU3D_FLUX_TJ;
+constexpr int kTK = 64;
+constexpr int kFluxThreads = 64 * (kTJ + 1);
+constexpr int kMarch = EU3D_FLUX_MARCH;
+constexpr int kXCD = 8; // L2 domains (XCDs); workgroups are dealt to them round-robin
+
+// Tile of a flux block.  Consecutive workgroup ids land on different XCDs,
+// each with its own L2; the tiles are renumbered so that every XCD gets a
+// contiguous run (k fastest, then j, then i) and the halo rows shared by
+// neighbouring tiles are read through the same L2.  false: padding block.
+struct FluxTile {
+    int k0, j0, i0;
+};
+__device__ __forceinline__ bool flux_tile(const Grid &g, FluxTile &t)
+{
+    const unsigned nbk = (g.nk + kTK - 1) / kTK, nbj = (g.nj + kTJ - 1) / kTJ, nbi = (g.ni + kMarch - 1) / kMarch;
+    const unsigned total = nbk * nbj * nbi, per = (total + kXCD - 1) / kXCD;
+    const unsigned b = blockIdx.x;
+    const unsigned tile = (b % kXCD) * per + b / kXCD;
+    if (tile >= total)
+        return false;
+    t.k0 = g.bc + (int)(tile % nbk) * kTK;
+    t.j0 = g.bc + (int)((tile / nbk) % nbj) * kTJ;
+    t.i0 = g.bc + (int)(tile / (nbk * nbj)) * kMarch;
+    return true;
+}
+
+static unsigned int flux_blocks(const Grid &g)
+{
+    const unsigned nbk = (g.nk + kTK - 1) / kTK, nbj = (g.nj + kTJ - 1) / kTJ, nbi = (g.ni + kMarch - 1) / kMarch;
+    const unsigned total = nbk * nbj * nbi;
+    return (total + kXCD - 1) / kXCD * kXCD;
+}
 
 template <int NS, int MODE>
 __global__ void __launch_bounds__(kFluxThreads) __attribute__((amdgpu_waves_per_eu(EU3D_FLUX_WAVES_PER_EU)))
@@ -348,11 +387,12 @@ __global__ void __launch_bounds__(kFluxThreads) __attribute__((amdgpu_waves_per_
     constexpr int NZF = kTJ * (kTK + 1); // z faces per plane
     static_assert(NZF <= NYF, "LDS face buffer sized for the y faces");
     __shared__ double face[NV * NYF];
+    FluxTile tile;
+    if (!flux_tile(g, tile))
+        return;
     const int t = threadIdx.x;
     const long long N = g.N;
-    const int k0 = g.bc + blockIdx.x * kTK;
-    const int j0 = g.bc + blockIdx.y * kTJ;
-    const int i0 = g.bc + blockIdx.z * kMarch;
+    const int k0 = tile.k0, j0 = tile.j0, i0 = tile.i0;
     const int i1 = min(i0 + kMarch, g.ni + g.bc);
     const int kend = g.nk + g.bc, jend = g.nj + g.bc;
     // the cell this thread owns
@@ -1192,7 +1232,7 @@ template <int NS, class MT>
 static void launch_flux(eu3d_ctx *ctx, int mode, double dt)
 {
     const Grid &g = ctx->g;
-    dim3 grid((g.nk + kTK - 1) / kTK, (g.nj + kTJ - 1) / kTJ, (g.ni + kMarch - 1) / kMarch);
+    const dim3 grid(flux_blocks(g));
     switch (mode) {
     case 0:
         k_flux_tile<NS, 0><<<grid, kFluxThreads, 0, ctx->stream>>>(g, ctx->prim, ctx->CS, ctx->CS0, ctx->x, ctx->y,
