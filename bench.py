@@ -34,6 +34,26 @@ FLUX_BYTES_PER_NODE = 336
 RK_EXTRA_BYTES_PER_NODE = 104
 
 
+def surrogate_box():
+    """One chemistry call on 2^20 seeded stiff 0-D states (the training
+    distribution of the committed model, dt = 3e-8 s): FP64 adaptive
+    trapezoid vs the BF16 network with FP64 fallback, device time of each
+    call (HIP events) and the error of the accepted cells."""
+    import importlib
+    sur = importlib.import_module("eu3d-gpu_amd.surrogate")
+    if not os.path.exists(sur.DEFAULT_MODEL):
+        return None
+    names = sur.species_names(os.path.join(REPO, "eu3d-gpu_amd", "input", "H2_JA919.txt"))
+    net = sur.Network.load_file(sur.DEFAULT_MODEL, max_batch=65536)
+    r = sur.evaluate_box(net, names, state_kw=dict(sur.TRAIN_STATES))
+    net.close()
+    r["workload"] = "chemistry call on 128x128x64 seeded 0-D H2-air states (T 1000-3000 K, p 0.1-30 atm)"
+    r["fallback_causes"] = {k: v for k, v in r.pop("fail").items() if k.startswith("fail_")}
+    print(f"bench: surrogate chemistry {r['fp64_ms']:.1f} ms -> {r['surrogate_ms']:.1f} ms", file=sys.stderr,
+          flush=True)
+    return r
+
+
 def eu3d_pkg():
     import importlib
     return importlib.import_module("eu3d-gpu_amd")
@@ -96,6 +116,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--ctrl", default=None, help="ctrl file (default: input/ctrl-3dSBI.txt of the package)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-surrogate", action="store_true",
+                    help="skip the chemistry-call comparison FP64 vs the committed surrogate")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device once (torch) and exit: the known-size kernel that "
@@ -200,6 +222,8 @@ def main():
         "surrogate_speedup": None,
     }
     s.close()
+    if not args.no_surrogate:
+        out["surrogate_speedup"] = surrogate_box()
     if not args.no_cpu_baseline:
         threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
         print(f"bench: cpu baseline with {threads} threads", file=sys.stderr, flush=True)

@@ -40,7 +40,7 @@ struct ChemCounters {
     // learned surrogate: routed cells, FP64 fallbacks and their causes
     unsigned int sur_count;
     unsigned int fb_count;
-    unsigned int fb_fetch;
+    unsigned int pad1;
     unsigned int pad2;
     unsigned long long fail_hull, fail_elem, fail_neg, fail_T;
 };
@@ -51,6 +51,7 @@ struct SurDev {
     double tol_T, tol_elem, y_floor;
     float lo[32], hi[32];
     double proj[16 * 16];
+    double rate_dt, elem_abs;
 };
 
 struct TimedKind {
@@ -75,7 +76,7 @@ struct eu3d_ctx {
     eu3d_mlp *mlp;
     int sur_min;
     SurDev *dsur;
-    int *sur_list, *fb_list;
+    int *sur_list;
     unsigned char *sur_ok;
     int sur_ok_cap;
     bool has_state, cms_valid;
@@ -568,7 +569,6 @@ __global__ void k_chem_reset(ChemCounters *c)
     c->max_nchem = 0;
     c->sur_count = 0;
     c->fb_count = 0;
-    c->fb_fetch = 0;
     c->fail_hull = 0;
     c->fail_elem = 0;
     c->fail_neg = 0;
@@ -791,7 +791,9 @@ __device__ __forceinline__ void chem_rest(const Grid &g, const MT &m, long long 
 }
 
 // Compacted: waves fetch 64 queued cells at a time until the queue is drained.
-template <int NS, class MT>
+// FB: the queue also holds cells the surrogate rejected, as ~c; those run
+// every sub-step from their unchanged state.
+template <int NS, class MT, bool FB = false>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(EU3D_CHEM_WAVES_PER_EU)))
     k_chem_queue(Grid g, const DevMech *__restrict__ mp, double dtm, double *__restrict__ CS,
                  double *__restrict__ prim, double *__restrict__ T, double *__restrict__ E, double *__restrict__ C,
@@ -810,7 +812,15 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(EU3
         unsigned int q = base + lane;
         if (q < total) {
             long long c = queue[q];
-            chem_rest<NS>(g, mech_ref<MT>(mp), c, nchem[c], dtm, CS, prim, T, E, C, CMS, keep_cms);
+            if (FB && c < 0) {
+                c = ~c;
+                const auto &m = mech_ref<MT>(mp);
+                int n = chem_step_cell<NS, true>(g, m, c, dtm, 0, CS, prim, T, E, C, CMS, keep_cms);
+                if (n > 1)
+                    chem_rest<NS>(g, m, c, n, dtm, CS, prim, T, E, C, CMS, keep_cms);
+            } else {
+                chem_rest<NS>(g, mech_ref<MT>(mp), c, nchem[c], dtm, CS, prim, T, E, C, CMS, keep_cms);
+            }
         }
     }
 }
@@ -840,7 +850,8 @@ using eu3d_mlp_internal::f2bf;
 using eu3d_mlp_internal::kIn;
 using eu3d_mlp_internal::kOut;
 
-// Raw features of cell c: T, ln p, ln(Y_s + 1e-12), ln dt.
+// Raw features of cell c: T, ln p, ln(max(Y_s, 0) + 1e-12), ln dt (the FP64
+// trapezoid leaves small negative mass fractions, which map to ln 1e-12).
 template <int NS>
 __device__ __forceinline__ void sur_features(long long N, long long c, double dt, const double *__restrict__ prim,
                                              const double *__restrict__ T, float (&f)[NS + 3])
@@ -849,7 +860,7 @@ __device__ __forceinline__ void sur_features(long long N, long long c, double dt
     f[1] = (float)log(prim[c]);
 #pragma unroll
     for (int s = 0; s < NS; s++)
-        f[2 + s] = (float)log(prim[(7 + s) * N + c] + 1e-12);
+        f[2 + s] = (float)log(fmax(prim[(7 + s) * N + c], 0.0) + 1e-12);
     f[NS + 2] = (float)log(dt);
 }
 
@@ -886,14 +897,15 @@ __global__ void __launch_bounds__(kBlock)
 
 // Apply the network's (dY, dT) to the listed cells at constant density,
 // momentum and internal energy, with the acceptance checks; a failing cell is
-// left unchanged and appended to fb_list for the FP64 integrator.
+// left unchanged and appended to the stiff queue (as ~c) for the FP64
+// integrator, which then balances them with the other stiff cells.
 template <int NS, class MT>
 __global__ void __launch_bounds__(kBlock)
     k_sur_apply(Grid g, const DevMech *__restrict__ mp, const SurDev *__restrict__ sp, const int *__restrict__ list,
                 int count, double dt, const float *__restrict__ Y, const float *__restrict__ norm,
                 const unsigned char *__restrict__ ok, double *__restrict__ CS, double *__restrict__ prim,
                 double *__restrict__ T, double *__restrict__ E, double *__restrict__ C, double *__restrict__ CMS,
-                int keep_cms, int *__restrict__ fb_list, ChemCounters *cnt)
+                int keep_cms, int *__restrict__ queue, ChemCounters *cnt)
 {
     const int row = blockIdx.x * blockDim.x + threadIdx.x;
     if (row >= count)
@@ -910,11 +922,12 @@ __global__ void __launch_bounds__(kBlock)
     const double U = prim[2 * N + c], V = prim[3 * N + c], W = prim[4 * N + c];
     const float *yr = Y + (long long)row * kOut;
     const float *om = norm + 2 * kIn, *os = norm + 2 * kIn + kOut;
+    const double scale = sp->rate_dt > 0.0 ? dt / sp->rate_dt : 1.0;
     double dY[NS];
 #pragma unroll
     for (int s = 0; s < NS; s++)
-        dY[s] = (double)(yr[s] * os[s] + om[s]);
-    const double dT = (double)(yr[NS] * os[NS] + om[NS]);
+        dY[s] = (double)(yr[s] * os[s] + om[s]) * scale;
+    const double dT = (double)(yr[NS] * os[NS] + om[NS]) * scale;
     int fail = ok[row] ? 0 : 1;
     // element-conservation projection
     double dYp[NS], corr = 0.0, mag = 0.0;
@@ -928,7 +941,7 @@ __global__ void __launch_bounds__(kBlock)
         corr += fabs(a - dY[s]);
         mag += fabs(a);
     }
-    if (!fail && !(corr <= sp->tol_elem * (mag + 1e-6)))
+    if (!fail && !(corr <= sp->tol_elem * mag + sp->elem_abs))
         fail = 2;
     double cs[NS + 4], Dn = 0.0;
 #pragma unroll
@@ -950,7 +963,10 @@ __global__ void __launch_bounds__(kBlock)
         if (!(Tg > 200.0 && Tg < 6000.0))
             Tg = T0;
         recover<NS>(m, cs, Tg, E0, st);
-        if (!(isfinite(st.T) && st.T > 0.0 && fabs(st.T - (T0 + dT)) <= sp->tol_T))
+        // a usable state (finite positive T, p, c, gamma > 1) whose T agrees
+        // with the network's own temperature change
+        if (!(isfinite(st.T) && st.T > 200.0 && st.T < 6000.0 && st.P > 0.0 && st.G > 1.0 && isfinite(st.C) &&
+              st.C > 0.0 && fabs(st.T - (T0 + dT)) <= sp->tol_T))
             fail = 4;
     }
     if (fail) {
@@ -959,7 +975,8 @@ __global__ void __launch_bounds__(kBlock)
                                   : fail == 3 ? &cnt->fail_neg
                                               : &cnt->fail_T;
         atomicAdd(why, 1ull);
-        fb_list[atomicAdd(&cnt->fb_count, 1u)] = (int)c;
+        atomicAdd(&cnt->fb_count, 1u);
+        queue[atomicAdd(&cnt->queue_count, 1u)] = ~(int)c;
         return;
     }
 #pragma unroll
@@ -971,34 +988,6 @@ __global__ void __launch_bounds__(kBlock)
             CMS[s * N + c] = (cs[s] - rho[s]) / dt;
     }
     store_state<NS>(N, c, st, prim, T, E, C);
-}
-
-// FP64 integration (all sub-steps) of the cells the surrogate rejected.
-template <int NS, class MT>
-__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(EU3D_CHEM_WAVES_PER_EU)))
-    k_chem_fallback(Grid g, const DevMech *__restrict__ mp, double dtm, double *__restrict__ CS,
-                    double *__restrict__ prim, double *__restrict__ T, double *__restrict__ E,
-                    double *__restrict__ C, double *__restrict__ CMS, const int *__restrict__ fb_list,
-                    ChemCounters *cnt, int keep_cms)
-{
-    const unsigned int total = cnt->fb_count;
-    const int lane = threadIdx.x & 63;
-    for (;;) {
-        unsigned int base = 0;
-        if (lane == 0)
-            base = atomicAdd(&cnt->fb_fetch, 64u);
-        base = __shfl(base, 0, 64);
-        if (base >= total)
-            break;
-        unsigned int q = base + lane;
-        if (q < total) {
-            const auto &m = mech_ref<MT>(mp);
-            long long c = fb_list[q];
-            int n = chem_step_cell<NS, true>(g, m, c, dtm, 0, CS, prim, T, E, C, CMS, keep_cms);
-            if (n > 1)
-                chem_rest<NS>(g, m, c, n, dtm, CS, prim, T, E, C, CMS, keep_cms);
-        }
-    }
 }
 
 // Cells at rest from (T, p, Y), as FieldInitial builds them
@@ -1324,14 +1313,10 @@ static void launch_chem_sur(eu3d_ctx *ctx, double dt, int flags, int *rc)
         }
         k_sur_apply<NS, MT><<<blocks_for(count), kBlock, 0, ctx->stream>>>(
             g, ctx->mech, ctx->dsur, ctx->sur_list + off, count, dt, v.Y, v.norm, ctx->sur_ok, ctx->CS, ctx->prim,
-            ctx->T, ctx->E, ctx->C, ctx->CMS, keep, ctx->fb_list, ctx->cnt);
+            ctx->T, ctx->E, ctx->C, ctx->CMS, keep, ctx->queue, ctx->cnt);
     }
-    k_chem_queue<NS, MT><<<ctx->ncu * 4, kBlock, 0, ctx->stream>>>(g, ctx->mech, dt, ctx->CS, ctx->prim, ctx->T,
-                                                                   ctx->E, ctx->C, ctx->CMS, ctx->nchem, ctx->queue,
-                                                                   ctx->cnt, keep);
-    k_chem_fallback<NS, MT><<<ctx->ncu * 4, kBlock, 0, ctx->stream>>>(g, ctx->mech, dt, ctx->CS, ctx->prim, ctx->T,
-                                                                      ctx->E, ctx->C, ctx->CMS, ctx->fb_list,
-                                                                      ctx->cnt, keep);
+    k_chem_queue<NS, MT, true><<<ctx->ncu * 4, kBlock, 0, ctx->stream>>>(
+        g, ctx->mech, dt, ctx->CS, ctx->prim, ctx->T, ctx->E, ctx->C, ctx->CMS, ctx->nchem, ctx->queue, ctx->cnt, keep);
 }
 
 template <int NS, class MT>
@@ -1440,7 +1425,7 @@ int eu3d_destroy(eu3d_ctx *ctx)
         (void)hipEventDestroy(e);
     void *ptrs[] = {ctx->mech,  ctx->x,      ctx->y,     ctx->z,       ctx->prim,    ctx->T,
                     ctx->E,     ctx->C,      ctx->CS,    ctx->CS0,     ctx->CMS,     ctx->scratch,
-                    ctx->nchem, ctx->queue,  ctx->cnt,   ctx->dsur,    ctx->sur_list, ctx->fb_list,
+                    ctx->nchem, ctx->queue,  ctx->cnt,   ctx->dsur,    ctx->sur_list,
                     ctx->sur_ok};
     for (void *p : ptrs)
         if (p)
@@ -1809,7 +1794,8 @@ int eu3d_set_surrogate(eu3d_ctx *ctx, eu3d_mlp *mlp, const eu3d_surrogate_params
     eu3d_mlp_shape(mlp, &n_in, nullptr, &n_out, &max_batch);
     if (n_in != ctx->ns + 3 || n_out != ctx->ns + 1)
         FAIL(EU3D_ERR_ARG, "network shape must be ns+3 -> ns+1");
-    if (p->min_nchem < 1 || !(p->tol_T > 0) || !(p->tol_elem > 0) || !(p->y_floor >= 0))
+    if (p->min_nchem < 1 || !(p->tol_T > 0) || !(p->tol_elem > 0) || !(p->y_floor >= 0) || !(p->rate_dt >= 0) ||
+        !(p->elem_abs >= 0))
         FAIL(EU3D_ERR_ARG, "surrogate parameters out of range");
     SurDev h{};
     h.min_nchem = p->min_nchem;
@@ -1818,6 +1804,8 @@ int eu3d_set_surrogate(eu3d_ctx *ctx, eu3d_mlp *mlp, const eu3d_surrogate_params
     h.tol_T = p->tol_T;
     h.tol_elem = p->tol_elem;
     h.y_floor = p->y_floor;
+    h.rate_dt = p->rate_dt;
+    h.elem_abs = p->elem_abs;
     for (int q = 0; q < 32; q++) {
         h.lo[q] = p->hull_lo[q];
         h.hi[q] = p->hull_hi[q];
@@ -1830,8 +1818,6 @@ int eu3d_set_surrogate(eu3d_ctx *ctx, eu3d_mlp *mlp, const eu3d_surrogate_params
     CK(hipMemcpy(ctx->dsur, &h, sizeof h, hipMemcpyHostToDevice));
     if (!ctx->sur_list)
         CK(hipMalloc(&ctx->sur_list, sizeof(int) * ctx->n_int));
-    if (!ctx->fb_list)
-        CK(hipMalloc(&ctx->fb_list, sizeof(int) * ctx->n_int));
     if (ctx->sur_ok_cap < max_batch) {
         if (ctx->sur_ok)
             CK(hipFree(ctx->sur_ok));
@@ -2022,7 +2008,7 @@ int eu3d_health(eu3d_ctx *ctx, eu3d_health_info *out)
     out->invalid_nchem = (long long)c.invalid;
     out->nonfinite = (long long)c.nonfinite;
     out->max_nchem = c.max_nchem;
-    out->stiff_cells = (long long)c.queue_count;
+    out->stiff_cells = (long long)(c.queue_count - c.fb_count);
     out->total_substeps = (long long)c.substeps;
     if (c.invalid || c.nonfinite) {
         ctx->err = "invalid Nchem or non-finite state in the last chemistry call";

@@ -34,7 +34,7 @@ else:
 load, ptr, check, EU3DError = _pkg.load, _pkg.ptr, _pkg.check, _pkg.EU3DError
 _vp = ctypes.c_void_p
 
-Y_EPS = 1e-12          # ln(Y + Y_EPS) feature
+Y_EPS = 1e-12          # ln(max(Y, 0) + Y_EPS) feature
 ATOMIC_MASS = {"H": 1.00794, "O": 15.9994, "N": 14.0067, "C": 12.011, "AR": 39.948, "HE": 4.002602}
 DEFAULT_MODEL = os.path.join(HERE, "input", "surrogate-H2_JA919.npz")
 
@@ -122,12 +122,12 @@ def sample_states(n, names, mw, seed, T_range=(1000.0, 3000.0), p_atm=(0.05, 50.
 
 
 def features(T, p, Y, dt):
-    """Raw features as the device builds them: T, ln p, ln(Y + 1e-12), ln dt."""
+    """Raw features as the device builds them: T, ln p, ln(max(Y, 0) + 1e-12), ln dt."""
     n = len(T)
     f = np.empty((n, Y.shape[1] + 3), dtype=np.float32)
     f[:, 0] = T
     f[:, 1] = np.log(p)
-    f[:, 2:-1] = np.log(Y + Y_EPS)
+    f[:, 2:-1] = np.log(np.maximum(Y, 0.0) + Y_EPS)
     f[:, -1] = np.log(np.broadcast_to(dt, (n,)))
     return f
 
@@ -152,6 +152,7 @@ class Network:
         self.n_in, self.n_out, self.hidden, self.max_batch = n_in, n_out, hidden, max_batch
         self.hull_lo = np.full(n_in, -np.inf, np.float32)
         self.hull_hi = np.full(n_in, np.inf, np.float32)
+        self.rate_dt = 0.0  # > 0: targets are changes per rate_dt seconds
 
     def _chk(self, rc, what):
         if rc != 0:
@@ -219,7 +220,7 @@ class Network:
     def save(self, path):
         im, isd, om, osd = self.get_norm()
         np.savez(path, params=self.get_params(), in_mean=im, in_std=isd, out_mean=om, out_std=osd,
-                 hull_lo=self.hull_lo, hull_hi=self.hull_hi,
+                 hull_lo=self.hull_lo, hull_hi=self.hull_hi, rate_dt=np.array(self.rate_dt),
                  shape=np.array([self.n_in, self.hidden, self.n_out], np.int64))
 
     @classmethod
@@ -230,13 +231,19 @@ class Network:
         net.set_params(z["params"])
         net.set_norm(z["in_mean"], z["in_std"], z["out_mean"], z["out_std"])
         net.hull_lo, net.hull_hi = z["hull_lo"].astype(np.float32), z["hull_hi"].astype(np.float32)
+        net.rate_dt = float(z["rate_dt"]) if "rate_dt" in z.files else 0.0
         return net
 
 
 # ---------------------------------------------------------------------------
 # solver side
 # ---------------------------------------------------------------------------
-def attach(ctx, net, names, min_nchem=2, tol_T=25.0, tol_elem=0.05, y_floor=1e-4, hull=None):
+# acceptance policy of the committed model (eu3d_surrogate_params)
+ACCEPT = dict(min_nchem=2, tol_T=25.0, tol_elem=0.1, elem_abs=5e-4, y_floor=1e-3)
+
+
+def attach(ctx, net, names, min_nchem=ACCEPT["min_nchem"], tol_T=ACCEPT["tol_T"], tol_elem=ACCEPT["tol_elem"],
+           y_floor=ACCEPT["y_floor"], elem_abs=ACCEPT["elem_abs"], hull=None):
     """eu3d_set_surrogate with the element projector of the species list and
     the network's training box (or hull=(lo, hi))."""
     L = load()
@@ -245,6 +252,8 @@ def attach(ctx, net, names, min_nchem=2, tol_T=25.0, tol_elem=0.05, y_floor=1e-4
     ns = len(names)
     prm = _pkg.SurrogateParams()
     prm.min_nchem, prm.tol_T, prm.tol_elem, prm.y_floor = min_nchem, tol_T, tol_elem, y_floor
+    prm.rate_dt = float(net.rate_dt)
+    prm.elem_abs = float(elem_abs)
     lo, hi = hull if hull is not None else (net.hull_lo, net.hull_hi)
     for q in range(32):
         prm.hull_lo[q] = float(lo[q]) if q < len(lo) else 0.0
@@ -306,7 +315,7 @@ def write_box_case(tmpdir, ni, nj, nk, name="samples"):
 
 
 def ignition_dataset(solver, names, mw, n_states, calls, seed, dt_range=(1e-9, 1e-6), min_nchem=2,
-                     T_range=(1000.0, 3000.0), log=None):
+                     T_range=(1000.0, 3000.0), log=None, state_kw=None):
     """Harvest (features, targets) from 0-D trajectories: the solver's interior
     cells are filled with seeded states (sample_states), then advanced by
     `calls` FP64 chemistry calls, each with its own log-uniform dt; every call
@@ -317,7 +326,7 @@ def ignition_dataset(solver, names, mw, n_states, calls, seed, dt_range=(1e-9, 1
     xs, ts = [], []
     for b0 in range(0, n_states, cap):
         n = cap  # every cell holds a sample (n_states rounds up to whole boxes)
-        T, p, Y = sample_states(n, names, mw, seed + b0, T_range=T_range)
+        T, p, Y = sample_states(n, names, mw, seed + b0, T_range=T_range, **(state_kw or {}))
         set_cells(ctx, T, p, Y)
         for c in range(calls):
             dt = float(np.exp(rng.uniform(np.log(dt_range[0]), np.log(dt_range[1]))))
@@ -329,26 +338,190 @@ def ignition_dataset(solver, names, mw, n_states, calls, seed, dt_range=(1e-9, 1
     return np.concatenate(xs), np.concatenate(ts)
 
 
-def train_recipe(out, n_states=1 << 18, calls=16, epochs=60, batch=65536, seed=20240601, log=print):
-    """The committed model's recipe: 0-D ignition data, normalization from the
-    set, Adam with a cosine learning rate, training box as the hull."""
+def march(solver, steps, chem, cfl=0.5, t0=0.0):
+    """The step loop of eu3d_run (CFL dt, SSP-RK3 advection) with the
+    chemistry call supplied by the caller: chem(ctx, dt).  Returns the time."""
+    L, ctx = load(), solver.ctx
+    final = solver.info()["final_time"]
+    t = t0
+    dt = ctypes.c_double()
+    for _ in range(steps):
+        check(L.eu3d_cfl_dt(ctx, cfl, t, final, ctypes.byref(dt)), ctx, "eu3d_cfl_dt")
+        check(L.eu3d_advance_rk3(ctx, dt.value), ctx, "eu3d_advance_rk3")
+        chem(ctx, dt.value)
+        t += dt.value
+    return t
+
+
+def znd_dataset(steps, min_nchem=2, ctrl=None, log=None):
+    """(features, targets) of every stiff cell of every step of the 1-D ZND
+    initiation (input/ctrl-1dZND.txt), FP64 chemistry."""
+    s = _pkg.Solver(ctrl or os.path.join(HERE, "input", "ctrl-1dZND.txt"), base_dir=HERE)
+    ns, n = s.ns, s.ni * s.nj * s.nk
+    xs, ts = [], []
+
+    def chem(ctx, dt):
+        x, t, _ = harvest_call(ctx, ns, dt, _pkg.CHEM_COMPACT | _pkg.CHEM_KEEP_CMS, min_nchem, n)
+        xs.append(x)
+        ts.append(t)
+
+    march(s, steps, chem)
+    s.close()
+    if log:
+        log(f"ZND harvest: {steps} steps, {sum(len(x) for x in xs)} rows")
+    return np.concatenate(xs), np.concatenate(ts)
+
+
+def front_index(s, p_ref=2.0 * 101325.0):
+    """Leading-shock cell of a 1-D run: the largest i with p > p_ref on the
+    centre line (interior indices)."""
+    s.sync()
+    p = s.interior(s.field("P"))[:, s.nj // 2, s.nk // 2]
+    idx = np.nonzero(p > p_ref)[0]
+    return int(idx.max()) if idx.size else -1
+
+
+def compare_znd(net, steps, names, ctrl=None, **attach_kw):
+    """FP64 vs surrogate over `steps` ZND steps from the same start: front
+    position (cells), peak temperature, species L-inf and the fallback
+    counts of the surrogate run."""
+    ctrl = ctrl or os.path.join(HERE, "input", "ctrl-1dZND.txt")
+    L = load()
+    runs = {}
+    for mode in ("fp64", "surrogate"):
+        s = _pkg.Solver(ctrl, base_dir=HERE)
+        flags = _pkg.CHEM_COMPACT | _pkg.CHEM_KEEP_CMS
+        routed = fallback = 0
+        if mode == "surrogate":
+            attach(s.ctx, net, names, **attach_kw)
+            flags |= _pkg.CHEM_SURROGATE
+
+        def chem(ctx, dt):
+            nonlocal routed, fallback
+            check(L.eu3d_chemistry(ctx, dt, flags), ctx, "eu3d_chemistry")
+            if mode == "surrogate":
+                st = stats(ctx)
+                routed += st["routed"]
+                fallback += st["fallback"]
+
+        error = None
+        try:
+            march(s, steps, chem)
+        except EU3DError as e:
+            error = str(e)
+        s.sync()
+        runs[mode] = dict(front=front_index(s), T=np.array(s.interior(s.field("T"))),
+                          Y=np.array(s.interior(s.field("Yi"))), routed=routed, fallback=fallback, error=error)
+        s.close()
+    a, b = runs["fp64"], runs["surrogate"]
+    if a["error"] or b["error"]:
+        return dict(steps=steps, error_fp64=a["error"], error_surrogate=b["error"], routed=b["routed"],
+                    fallback=b["fallback"])
+    return dict(steps=steps, front_fp64=a["front"], front_surrogate=b["front"],
+                front_shift_cells=abs(a["front"] - b["front"]),
+                peakT_fp64=float(a["T"].max()), peakT_surrogate=float(b["T"].max()),
+                peakT_rel_err=float(abs(b["T"].max() - a["T"].max()) / a["T"].max()),
+                T_linf=float(np.abs(a["T"] - b["T"]).max()), Y_linf=float(np.abs(a["Y"] - b["Y"]).max()),
+                routed=b["routed"], fallback=b["fallback"])
+
+
+def evaluate_box(net, names, n_side=(128, 128, 64), seed=777, dt=3e-8, min_nchem=2, state_kw=None,
+                 attach_kw=None):
+    """Held-out 0-D states in two boxes: FP64 chemistry vs the surrogate
+    chemistry on identical states -- device times of the two calls, error
+    of the accepted cells and the fallback counts."""
+    state_kw = state_kw or {}
+    attach_kw = attach_kw or {}
+    L = load()
+    with tempfile.TemporaryDirectory() as d:
+        ctrl = write_box_case(d, *n_side)
+        out = {}
+        states = None
+        for mode in ("fp64", "surrogate"):
+            s = _pkg.Solver(ctrl, base_dir=d)
+            if states is None:
+                mw = np.array(s.field("mw"), dtype=float)
+                states = sample_states(s.ni * s.nj * s.nk, names, mw, seed, **state_kw)
+            set_cells(s.ctx, *states)
+            flags = _pkg.CHEM_COMPACT
+            if mode == "surrogate":
+                attach(s.ctx, net, names, min_nchem=min_nchem, **attach_kw)
+                flags |= _pkg.CHEM_SURROGATE
+            check(L.eu3d_timing_enable(s.ctx, 1), s.ctx, "timing")
+            tm = _pkg.Timing()
+            check(L.eu3d_timing_get(s.ctx, tm), s.ctx, "timing")
+            check(L.eu3d_chemistry(s.ctx, dt, flags), s.ctx, "eu3d_chemistry")
+            check(L.eu3d_synchronize(s.ctx), s.ctx, "sync")
+            check(L.eu3d_timing_get(s.ctx, tm), s.ctx, "timing")
+            st = stats(s.ctx) if mode == "surrogate" else None
+            rc, h = s.health()
+            s.sync()
+            out[mode] = dict(ms=tm.chem_ms, T=np.array(s.interior(s.field("T"))).reshape(-1),
+                             Y=np.array(s.interior(s.field("Yi"))).reshape(-1, len(names)),
+                             nchem=np.array(s.interior(s.nchem())).reshape(-1), stats=st,
+                             substeps=h.total_substeps)
+            s.close()
+    a, b = out["fp64"], out["surrogate"]
+    changed = (np.any(a["Y"] != b["Y"], axis=1)) | (a["T"] != b["T"])
+    dT = np.abs(a["T"] - b["T"])[changed]
+    dY = np.abs(a["Y"] - b["Y"])[changed]
+    st = b["stats"]
+    return dict(cells=int(a["T"].size), dt=dt, min_nchem=min_nchem,
+                stiff_cells=int(np.count_nonzero(a["nchem"] >= min_nchem)), fp64_substeps=int(a["substeps"]),
+                fp64_ms=a["ms"], surrogate_ms=b["ms"], chem_speedup=a["ms"] / b["ms"] if b["ms"] > 0 else None,
+                routed=st["routed"], fallback=st["fallback"], accepted=int(np.count_nonzero(changed)),
+                T_abs_err_median=float(np.median(dT)) if dT.size else 0.0,
+                T_abs_err_p99=float(np.percentile(dT, 99)) if dT.size else 0.0,
+                T_abs_err_p999=float(np.percentile(dT, 99.9)) if dT.size else 0.0,
+                T_abs_err_max=float(dT.max()) if dT.size else 0.0,
+                Y_abs_err_p99=float(np.percentile(dY.max(1), 99)) if dY.size else 0.0,
+                Y_linf=float(dY.max()) if dY.size else 0.0, fail=st)
+
+
+# 0-D states and time steps of the committed model's training set: around
+# the solver's CFL steps (1e-8 .. 1e-7 s at 0.05 - 0.1 mm), hot enough to
+# react, away from the states that need 1e5+ FP64 sub-steps per call.
+TRAIN_STATES = dict(T_range=(1000.0, 3000.0), p_atm=(0.1, 30.0), phi=(0.5, 2.0), radical_max=1e-3)
+TRAIN_DT = (3e-9, 1e-7)
+RATE_DT = 1e-8  # targets are learned as changes per 10 ns (scaled by RATE_DT / dt)
+
+
+def train_recipe(out, n_states=1 << 18, calls=8, znd_steps=3000, epochs=600, batch=16384, seed=20240601,
+                 log=print):
+    """The committed model's recipe: 0-D ignition data (TRAIN_STATES,
+    TRAIN_DT) plus the stiff cells of a ZND initiation run, normalization
+    from the set, Adam with a cosine learning rate, training box as the hull."""
     t0 = time.time()
+    names = species_names(os.path.join(HERE, "input", "H2_JA919.txt"))
     with tempfile.TemporaryDirectory() as d:
         side = 64
-        ctrl = write_box_case(d, side, side, max(3, (min(n_states, 1 << 18) + side * side - 1) // (side * side)))
+        ctrl = write_box_case(d, side, side, max(3, (n_states + side * side - 1) // (side * side)))
         solver = _pkg.Solver(ctrl, base_dir=d)
-        names = species_names(os.path.join(HERE, "input", "H2_JA919.txt"))
         mw = np.array(solver.field("mw"), dtype=float)
-        x, t = ignition_dataset(solver, names, mw, n_states, calls, seed, log=None)
+        x0, t0_ = ignition_dataset(solver, names, mw, n_states, calls, seed, dt_range=TRAIN_DT,
+                                   T_range=TRAIN_STATES["T_range"], log=None,
+                                   state_kw={k: v for k, v in TRAIN_STATES.items() if k != "T_range"})
         solver.close()
-    log(f"harvested {len(x)} rows in {time.time() - t0:.1f} s")
-    net = Network(x.shape[1], t.shape[1], max_batch=batch, seed=seed)
+    log(f"0-D harvest: {len(x0)} rows in {time.time() - t0:.1f} s")
+    t1 = time.time()
+    x1, t1_ = znd_dataset(znd_steps, log=log)
+    log(f"ZND harvest in {time.time() - t1:.1f} s")
+    x, t = np.concatenate([x0, x1]), np.concatenate([t0_, t1_])
+    keep = np.isfinite(x).all(1) & np.isfinite(t).all(1)
+    if not keep.all():
+        log(f"dropped {np.count_nonzero(~keep)} non-finite rows")
+        x, t = x[keep], t[keep]
+    t = t * (RATE_DT / np.exp(x[:, -1].astype(np.float64)))[:, None]
+    net = Network(x.shape[1], t.shape[1], max_batch=max(batch, 65536), seed=seed)
+    net.rate_dt = RATE_DT
     net.fit_norm(x, t)
     net.hull_lo, net.hull_hi = x.min(0), x.max(0)
     net.set_dataset(x, t)
-    t1 = time.time()
+    t2 = time.time()
     losses = net.train(epochs, batch, lr=2e-3, lr_end=2e-5, seed=seed)
-    log(f"trained {epochs} epochs in {time.time() - t1:.1f} s, loss {losses[0]:.4e} -> {losses[-1]:.4e}")
+    log(f"trained {epochs} epochs ({len(x)} rows, batch {batch}) in {time.time() - t2:.1f} s, "
+        f"loss {losses[0]:.4e} -> {losses[-1]:.4e}; by tenths: "
+        + " ".join(f"{losses[min(len(losses) - 1, i * len(losses) // 10)]:.3e}" for i in range(11)))
     net.save(out)
     return net, losses
 
@@ -359,11 +532,24 @@ def main():
     tr = sub.add_parser("train")
     tr.add_argument("--out", default=DEFAULT_MODEL)
     tr.add_argument("--states", type=int, default=1 << 18)
-    tr.add_argument("--calls", type=int, default=16)
-    tr.add_argument("--epochs", type=int, default=60)
+    tr.add_argument("--calls", type=int, default=8)
+    tr.add_argument("--znd-steps", type=int, default=3000)
+    tr.add_argument("--epochs", type=int, default=600)
+    ev = sub.add_parser("eval")
+    ev.add_argument("--model", default=DEFAULT_MODEL)
+    ev.add_argument("--znd-steps", type=int, default=2000)
+    ev.add_argument("--tol-T", type=float, default=ACCEPT["tol_T"])
     a = ap.parse_args()
     if a.cmd == "train":
-        train_recipe(a.out, n_states=a.states, calls=a.calls, epochs=a.epochs)
+        train_recipe(a.out, n_states=a.states, calls=a.calls, znd_steps=a.znd_steps, epochs=a.epochs)
+    else:
+        import json
+        names = species_names(os.path.join(HERE, "input", "H2_JA919.txt"))
+        net = Network.load_file(a.model, max_batch=65536)
+        box = evaluate_box(net, names, state_kw={k: v for k, v in TRAIN_STATES.items()},
+                           attach_kw=dict(tol_T=a.tol_T))
+        print(json.dumps({"box": box}), flush=True)
+        print(json.dumps({"znd": compare_znd(net, a.znd_steps, names, tol_T=a.tol_T)}), flush=True)
 
 
 if __name__ == "__main__":

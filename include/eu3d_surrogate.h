@@ -11,8 +11,8 @@
  *
  * Network: MLP n_in -> H -> H -> H -> n_out with GELU, BF16 weights and
  * activations, FP32 accumulation on MFMA, FP32 master weights, Adam.
- * Features (n_in = ns + 3): T [K], ln p [Pa], ln(Y_s + 1e-12) for each
- * species, ln dt [s].  Targets (n_out = ns + 1): Y_s(t+dt) - Y_s(t), and
+ * Features (n_in = ns + 3): T [K], ln p [Pa], ln(max(Y_s, 0) + 1e-12) for
+ * each species, ln dt [s].  Targets (n_out = ns + 1): Y_s(t+dt) - Y_s(t), and
  * T(t+dt) - T(t) [K].  Features and targets are normalized on the device
  * with the table set by eu3d_mlp_set_norm.
  *
@@ -74,11 +74,11 @@ int eu3d_mlp_train(eu3d_mlp *mlp, int epochs, int B, float lr, float lr_end, uns
  *  - every feature lies in [hull_lo, hull_hi] (the training set's box);
  *  - the element-conservation projection (dY -> proj . dY, proj = I - A^T
  *    (A A^T)^-1 A for the element mass-fraction matrix A) moves dY by at
- *    most tol_elem * (sum |proj . dY| + 1e-6) in the 1-norm;
+ *    most tol_elem * sum |proj . dY| + elem_abs in the 1-norm;
  *  - Y_s + dY_s >= -y_floor for every species (small negatives clip to 0);
- *  - the temperature recovered from the new composition at the cell's
- *    internal energy (the Newton of Update_after_CS) is finite and within
- *    tol_T of T + dT_net. */
+ *  - the state recovered from the new composition at the cell's internal
+ *    energy (the Newton of Update_after_CS) has 200 K < T < 6000 K, p > 0,
+ *    gamma > 1 and a finite sound speed, and T within tol_T of T + dT_net. */
 typedef struct {
     int min_nchem;      /* cells with predicted Nchem >= min_nchem (>= 1) use the network */
     double tol_T;       /* K */
@@ -86,12 +86,16 @@ typedef struct {
     double y_floor;     /* absolute mass fraction */
     float hull_lo[32], hull_hi[32]; /* raw feature box, first n_in entries used */
     double proj[16 * 16];           /* ns x ns, row-major */
+    double rate_dt;     /* > 0: the network predicts the changes per rate_dt seconds
+                           (targets scaled by rate_dt / dt); 0: changes over dt */
+    double elem_abs;    /* absolute part of the element check (mass fraction) */
 } eu3d_surrogate_params;
 
 /* Attach (mlp != NULL) or detach (mlp == NULL) a network; the network must
  * have n_in = ns + 3 and n_out = ns + 1 and live on the context's device.
  * The context does not own it.  EU3D_CHEM_SURROGATE requires
- * EU3D_CHEM_COMPACT. */
+ * EU3D_CHEM_COMPACT; rejected cells join the stiff-cell queue, so
+ * eu3d_health's stiff_cells still counts only the FP64 path's own. */
 int eu3d_set_surrogate(eu3d_ctx *ctx, eu3d_mlp *mlp, const eu3d_surrogate_params *params);
 
 /* Counters of the last chemistry call. */

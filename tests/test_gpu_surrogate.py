@@ -295,3 +295,17 @@ def test_surrogate_trained_updates_conserve_and_fall_back(box):
     err_T = np.abs(Ta[acc] - Tb[acc])
     print(f"routed {st['routed']} fallback {st['fallback']} accepted {np.count_nonzero(acc)} "
           f"|dT| median {np.median(err_T):.2f} K max {err_T.max():.2f} K")
+
+
+def test_committed_model_on_held_out_states():
+    """The committed model (input/surrogate-H2_JA919.npz, made by
+    surrogate.py train) on seeded held-out 0-D states of its training
+    distribution: most stiff cells accepted, small median temperature error."""
+    assert os.path.exists(sur.DEFAULT_MODEL)
+    net = sur.Network.load_file(sur.DEFAULT_MODEL, max_batch=8192)
+    assert net.rate_dt == sur.RATE_DT
+    r = sur.evaluate_box(net, NAMES, n_side=(32, 32, 16), seed=4242, state_kw=dict(sur.TRAIN_STATES))
+    print({k: v for k, v in r.items() if k != "fail"})
+    assert r["routed"] > 1000
+    assert r["fallback"] <= 0.1 * r["routed"]
+    assert r["T_abs_err_median"] < 5.0
