@@ -27,6 +27,8 @@ CHEM_COMPACT = 0x1
 CHEM_SKIP = 0x2
 CHEM_KEEP_CMS = 0x4
 CHEM_SURROGATE = 0x8
+BOUNDARY_RSBI = 0
+BOUNDARY_ODW = 1
 OPT_GENERIC_MECHANISM = 1
 OPT_GENERATED_AVAILABLE = 2
 
@@ -98,6 +100,8 @@ SIGNATURES = {
     "eu3d_health": (_i, [_vp, ctypes.POINTER(Health)]),
     "eu3d_synchronize": (_i, [_vp]),
     "eu3d_set_option": (_i, [_vp, _i, _i]),
+    "eu3d_set_boundary": (_i, [_vp, _i, _vp]),
+    "eu3d_get_boundary": (_i, [_vp, ctypes.POINTER(_i)]),
     "eu3d_get_option": (_i, [_vp, _i, ctypes.POINTER(_i)]),
     "eu3d_timing_get": (_i, [_vp, ctypes.POINTER(Timing)]),
     "eu3d_timing_enable": (_i, [_vp, _i]),
@@ -118,6 +122,7 @@ SIGNATURES = {
     "eu3d_solver_nchem": (ctypes.POINTER(_i), [_vp]),
     "eu3d_solver_dump": (_i, [_vp, ctypes.c_char_p, _i]),
     "eu3d_solver_error": (ctypes.c_char_p, [_vp]),
+    "eu3d_solver_restart": (_i, [_vp, ctypes.c_char_p, _d]),
     "eu3d_mlp_create": (_i, [_i, _i, _i, _i, ctypes.c_ulonglong, ctypes.POINTER(_vp)]),
     "eu3d_mlp_destroy": (_i, [_vp]),
     "eu3d_mlp_last_error": (ctypes.c_char_p, [_vp]),
@@ -224,6 +229,10 @@ class Solver:
 
     def dump(self, path, num=1):
         self._check(self.L.eu3d_solver_dump(self.h, path.encode(), num), "dump")
+
+    def restart(self, path, time=-1.0):
+        """Restore the state of an EU3D_STATE_V1_LE dump (eu3d_solver_restart)."""
+        self._check(self.L.eu3d_solver_restart(self.h, path.encode(), time), "restart")
 
     def info(self):
         v = np.zeros(6)

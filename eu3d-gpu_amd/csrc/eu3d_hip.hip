@@ -79,6 +79,9 @@ struct eu3d_ctx {
     int *sur_list;
     unsigned char *sur_ok;
     int sur_ok_cap;
+    // ghost-fill set (EU3D_BOUNDARY_*) and the inflow state of the ODW set
+    int boundary;
+    double *inflow; // nprim + 2: P D U V W H Gamma Y.. T E
     bool has_state, cms_valid;
     bool generated; // tables bit-identical to MechH2JA919: use the compile-time kernels
     bool generated_available;
@@ -205,8 +208,20 @@ __device__ __forceinline__ void copy_point(const Grid &g, int nprim, double *pri
     E[d] = E[s];
 }
 
-// x faces: left mirror copy (freestream), right mirror copy (outflow); j, k interior
-__global__ void __launch_bounds__(kBlock) k_ghost_x(Grid g, int nprim, double *prim, double *T, double *E)
+// A ghost point set to the inflow state (nprim primitives, T, E).
+__device__ __forceinline__ void set_point(const Grid &g, int nprim, double *prim, double *T, double *E, long long d,
+                                          const double *__restrict__ inflow)
+{
+    for (int f = 0; f < nprim; f++)
+        prim[f * g.N + d] = inflow[f];
+    T[d] = inflow[nprim];
+    E[d] = inflow[nprim + 1];
+}
+
+// x faces: left mirror copy (freestream), right mirror copy (outflow); j, k
+// interior.  With an inflow state (ODW set) the left side is set to it.
+__global__ void __launch_bounds__(kBlock)
+    k_ghost_x(Grid g, int nprim, double *prim, double *T, double *E, const double *__restrict__ inflow)
 {
     long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
     long long per = (long long)g.bc * g.nj * g.nk;
@@ -222,6 +237,10 @@ __global__ void __launch_bounds__(kBlock) k_ghost_x(Grid g, int nprim, double *p
     if (side == 0) {
         i = ii;
         is = 2 * g.bc - 1 - i;
+        if (inflow) {
+            set_point(g, nprim, prim, T, E, gidx(g, i, j, k), inflow);
+            return;
+        }
     } else {
         i = g.ni + g.bc + ii;
         is = 2 * (g.ni + g.bc) - 1 - i;
@@ -290,6 +309,27 @@ __global__ void __launch_bounds__(kBlock)
     prim[5 * N + d] = sH * 1000;
     E[d] = sE * 1000;
     prim[6 * N + d] = Cp / (Cp - Rgas);
+}
+
+// y faces of the ODW set: y- slip wall (mirror, v -> -v), y+ inflow; all i,
+// k interior.
+__global__ void __launch_bounds__(kBlock)
+    k_ghost_y_odw(Grid g, int nprim, double *prim, double *T, double *E, const double *__restrict__ inflow)
+{
+    long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    long long per = (long long)g.NI * g.bc * g.nk;
+    if (t >= 2 * per)
+        return;
+    int side = (int)(t / per);
+    long long r = t % per;
+    int k = (int)(r % g.nk) + g.bc;
+    r /= g.nk;
+    int jj = (int)(r % g.bc);
+    int i = (int)(r / g.bc);
+    if (side == 0)
+        copy_point(g, nprim, prim, T, E, gidx(g, i, jj, k), gidx(g, i, 2 * g.bc - 1 - jj, k), 3);
+    else
+        set_point(g, nprim, prim, T, E, gidx(g, i, g.nj + g.bc + jj, k), inflow);
 }
 
 // z faces: w -> -w mirror on both sides; all i, j
@@ -1425,7 +1465,7 @@ int eu3d_destroy(eu3d_ctx *ctx)
         (void)hipEventDestroy(e);
     void *ptrs[] = {ctx->mech,  ctx->x,      ctx->y,     ctx->z,       ctx->prim,    ctx->T,
                     ctx->E,     ctx->C,      ctx->CS,    ctx->CS0,     ctx->CMS,     ctx->scratch,
-                    ctx->nchem, ctx->queue,  ctx->cnt,   ctx->dsur,    ctx->sur_list,
+                    ctx->nchem, ctx->queue,  ctx->cnt,   ctx->dsur,    ctx->sur_list, ctx->inflow,
                     ctx->sur_ok};
     for (void *p : ptrs)
         if (p)
@@ -1652,11 +1692,51 @@ int eu3d_fill_ghosts(eu3d_ctx *ctx)
     Timer tm(ctx, K_GHOST);
     const Grid &g = ctx->g;
     long long nx = 2LL * g.bc * g.nj * g.nk;
-    k_ghost_x<<<blocks_for(nx), kBlock, 0, ctx->stream>>>(g, ctx->nprim, ctx->prim, ctx->T, ctx->E);
-    EU3D_DISPATCH_NS(ctx->ns, launch_ghost_y, ctx);
+    const bool odw = ctx->boundary == EU3D_BOUNDARY_ODW;
+    k_ghost_x<<<blocks_for(nx), kBlock, 0, ctx->stream>>>(g, ctx->nprim, ctx->prim, ctx->T, ctx->E,
+                                                          odw ? ctx->inflow : nullptr);
+    if (odw) {
+        long long ny = 2LL * g.NI * g.bc * g.nk;
+        k_ghost_y_odw<<<blocks_for(ny), kBlock, 0, ctx->stream>>>(g, ctx->nprim, ctx->prim, ctx->T, ctx->E,
+                                                                  ctx->inflow);
+    } else {
+        EU3D_DISPATCH_NS(ctx->ns, launch_ghost_y, ctx);
+    }
     long long nz = 2LL * g.NI * g.NJ * g.bc;
     k_ghost_z<<<blocks_for(nz), kBlock, 0, ctx->stream>>>(g, ctx->nprim, ctx->prim, ctx->T, ctx->E);
     CK(hipGetLastError());
+    return EU3D_OK;
+}
+
+int eu3d_set_boundary(eu3d_ctx *ctx, int mode, const double *inflow)
+{
+    if (!ctx)
+        return EU3D_ERR_ARG;
+    if (mode == EU3D_BOUNDARY_RSBI) {
+        ctx->boundary = mode;
+        return EU3D_OK;
+    }
+    if (mode != EU3D_BOUNDARY_ODW || !inflow)
+        FAIL(EU3D_ERR_ARG, "unknown boundary set or missing inflow state");
+    const int n = ctx->nprim + 2;
+    for (int q = 0; q < n; q++)
+        if (!std::isfinite(inflow[q]))
+            FAIL(EU3D_ERR_ARG, "inflow state must be finite");
+    if (!(inflow[0] > 0 && inflow[1] > 0 && inflow[ctx->nprim] > 0))
+        FAIL(EU3D_ERR_ARG, "inflow p, rho and T must be > 0");
+    CK(hipStreamSynchronize(ctx->stream));
+    if (!ctx->inflow)
+        CK(hipMalloc(&ctx->inflow, sizeof(double) * n));
+    CK(hipMemcpy(ctx->inflow, inflow, sizeof(double) * n, hipMemcpyHostToDevice));
+    ctx->boundary = mode;
+    return EU3D_OK;
+}
+
+int eu3d_get_boundary(eu3d_ctx *ctx, int *mode)
+{
+    if (!ctx || !mode)
+        return EU3D_ERR_ARG;
+    *mode = ctx->boundary;
     return EU3D_OK;
 }
 

@@ -567,6 +567,54 @@ void Flowfield::ic_znd(const Reaction &r)
             }
 }
 
+// Oblique detonation in the wedge frame ("ODW"): x along the wedge surface
+// (wall y = ya); free stream of H2-air at equivalence ratio phi, P_pre, T_pre
+// and Mach M_wave meeting the wall at wedge_deg (u = V cos, v = -V sin);
+// uniform everywhere, the shock and detonation develop from the wall.
+void Flowfield::ic_odw(const Reaction &r)
+{
+    double phi = input.getdoubleDefault("phi", 1.0);
+    double deg = input.getdoubleDefault("wedge_deg", 25.0);
+    double theta = deg * 3.14159265358979323846 / 180.0;
+    double mw_u = 2.0 * phi * r.Mw[5] + r.Mw[4] + 3.76 * r.Mw[8];
+    std::vector<double> Y(NS, 0.0), Cpi(NS);
+    Y[5] = 2.0 * phi * r.Mw[5] / mw_u;
+    Y[4] = r.Mw[4] / mw_u;
+    Y[8] = 3.76 * r.Mw[8] / mw_u;
+    double sYM = 0;
+    for (int q = 0; q < NS; q++)
+        sYM += (Y[q] / r.Mw[q]);
+    double Tf = T_pre, Pf = P_pre;
+    double Wavf = 1.0 / sYM;
+    double Rg = r.R * 1000 / Wavf;
+    double Df = Pf / (Rg * Tf);
+    double sCp = 0;
+    for (int q = 0; q < NS; q++)
+        Cpi[q] = r.GetCpi(Tf, r.Ri[q], q);
+    for (int q = 0; q < NS; q++)
+        sCp += (Cpi[q] * Y[q]);
+    double Cpf = sCp * 1000;
+    double sYM2 = 0;
+    for (int q = 0; q < NS; q++)
+        sYM2 += (Y[q] / r.Mw[q]);
+    double G = Cpf / (Cpf - r.R * sYM2 * 1000);
+    double Vf = M_wave * std::sqrt(G * Pf / Df);
+    double u = Vf * std::cos(theta), v = -Vf * std::sin(theta);
+    for (int i = 0; i < NI; i++)
+        for (int j = 0; j < NJ; j++)
+            for (int k = 0; k < NK; k++) {
+                size_t id = idx(i, j, k);
+                for (int q = 0; q < NS; q++)
+                    Yi[id * NS + q] = Y[q];
+                U[id] = u;
+                V[id] = v;
+                W[id] = 0.0;
+                P[id] = Pf;
+                T[id] = Tf;
+            }
+    boundary = EU3D_BOUNDARY_ODW;
+}
+
 // Flowfield::FieldInitial (src/Flowfield.cpp:193-264) for the selected case
 void Flowfield::FieldInitial(const Reaction &r)
 {
@@ -578,6 +626,8 @@ void Flowfield::FieldInitial(const Reaction &r)
         ic_sbi(r);
     else if (case_name == "ZND")
         ic_znd(r);
+    else if (case_name == "ODW")
+        ic_odw(r);
     else
         throw Error(EU3D_ERR_ARG, "unknown case " + case_name);
     const double R = r.R;
@@ -624,6 +674,14 @@ void Flowfield::FieldInitial(const Reaction &r)
                 cs[NS + 2] = d * w;
                 cs[NS + 3] = d * E[id] + 0.5 * ((u * u) + (v * v) + (w * w)) * d;
             }
+    if (boundary == EU3D_BOUNDARY_ODW) { // inflow = the free stream as the interior holds it
+        size_t id = idx(bc, bc, bc);
+        inflow = {P[id], D[id], U[id], V[id], W[id], H[id], Gamma[id]};
+        for (int s = 0; s < NS; s++)
+            inflow.push_back(Yi[id * NS + s]);
+        inflow.push_back(T[id]);
+        inflow.push_back(E[id]);
+    }
 }
 
 eu3d_state Flowfield::HostState()
@@ -743,6 +801,8 @@ void Euler::Attach()
         TwoDim.ctx = nullptr;
         throw Error(rc, msg);
     }
+    if (TwoDim.boundary != EU3D_BOUNDARY_RSBI)
+        check(eu3d_set_boundary(TwoDim.ctx, TwoDim.boundary, TwoDim.inflow.data()), TwoDim.ctx, "eu3d_set_boundary");
     TwoDim.Upload();
 }
 
@@ -777,7 +837,8 @@ void Euler::SyncOutput() { TwoDim.Download(); }
 // EU3D_STATE_V1_LE dump (src/Euler.cpp:1539-1598), single process
 void Euler::WriteStateDump(const std::string &p, int num)
 {
-    SyncOutput();
+    if (TwoDim.ctx)
+        SyncOutput();
     std::FILE *f = std::fopen(p.c_str(), "wb");
     if (!f)
         throw Error(EU3D_ERR_ARG, "cannot open dump " + p);
@@ -819,6 +880,86 @@ void Euler::WriteStateDump(const std::string &p, int num)
             }
     if (std::fclose(f) != 0)
         throw Error(EU3D_ERR_ARG, "dump write failed " + p);
+}
+
+// Restart from an EU3D_STATE_V1_LE dump of the same grid and mechanism (no
+// reference counterpart: the reference only writes the dump).  Interior
+// state, Nchem, CMS and the iteration counter come from the file; the time
+// is restart_time, or iteration * dt for a fixed-dt run when restart_time < 0.
+// Ghost cells are refilled by the next step's ghost fill.
+void Euler::ReadStateDump(const std::string &p, double restart_time)
+{
+    std::FILE *f = std::fopen(p.c_str(), "rb");
+    if (!f)
+        throw Error(EU3D_ERR_ARG, "cannot open dump " + p);
+    auto line = [&]() {
+        std::string l;
+        int c;
+        while ((c = std::fgetc(f)) != EOF && c != '\n')
+            l.push_back((char)c);
+        if (c == EOF)
+            throw Error(EU3D_ERR_ARG, "truncated dump header " + p);
+        return l;
+    };
+    try {
+        if (line() != "EU3D_STATE_V1_LE")
+            throw Error(EU3D_ERR_ARG, "not an EU3D_STATE_V1_LE dump: " + p);
+        const std::string h = line();
+        int v[17];
+        double final_time = 0, dtm = 0;
+        int got = std::sscanf(h.c_str(), "%d %d %d %d %d %d %d %d %d %d %d %d %d %d %d %d %d %lf %lf", &v[0], &v[1],
+                              &v[2], &v[3], &v[4], &v[5], &v[6], &v[7], &v[8], &v[9], &v[10], &v[11], &v[12], &v[13],
+                              &v[14], &v[15], &v[16], &final_time, &dtm);
+        const int NS = React.NS, bc = Mymesh.bc;
+        if (got != 19)
+            throw Error(EU3D_ERR_ARG, "bad dump header: " + h);
+        if (v[0] != NS || v[4] != Mymesh.ni || v[5] != Mymesh.nj || v[6] != Mymesh.nk)
+            throw Error(EU3D_ERR_ARG, "dump " + p + " is for another grid or species count");
+        if (v[10] != 1 || v[11] != 1 || v[12] != 1)
+            throw Error(EU3D_ERR_ARG, "dump " + p + " is one block of a decomposed run");
+        const size_t ncol = 17 + 3 * NS + (NS + 4) + NS + 1;
+        std::istringstream names(line());
+        size_t nnames = 0;
+        for (std::string w; names >> w;)
+            nnames++;
+        if (nnames != ncol)
+            throw Error(EU3D_ERR_ARG, "dump " + p + " has an unexpected column list");
+        Flowfield &F = TwoDim;
+        std::vector<double> row(ncol);
+        for (int i = bc; i < bc + Mymesh.ni; i++)
+            for (int j = bc; j < bc + Mymesh.nj; j++)
+                for (int k = bc; k < bc + Mymesh.nk; k++) {
+                    if (std::fread(row.data(), sizeof(double), ncol, f) != ncol)
+                        throw Error(EU3D_ERR_ARG, "truncated dump " + p);
+                    if (row[0] != Mymesh.xnode[i] || row[1] != Mymesh.ynode[j] || row[2] != Mymesh.znode[k])
+                        throw Error(EU3D_ERR_ARG, "dump " + p + ": node coordinates differ from the grid");
+                    const size_t id = F.idx(i, j, k);
+                    double *dst[] = {&F.D[id],   &F.T[id], &F.U[id],  &F.V[id], &F.W[id],     &F.E[id], &F.Wav[id],
+                                     &F.Rgas[id], &F.Cp[id], &F.P[id], &F.H[id], &F.Gamma[id], &F.C[id], &F.Ma[id]};
+                    size_t c = 3;
+                    for (double *d : dst)
+                        *d = row[c++];
+                    for (int s = 0; s < NS; s++)
+                        F.Di[id * NS + s] = row[c++];
+                    for (int s = 0; s < NS; s++)
+                        F.Yi[id * NS + s] = row[c++];
+                    for (int s = 0; s < NS; s++)
+                        F.Mc[id * NS + s] = row[c++];
+                    for (int s = 0; s < NS + 4; s++)
+                        F.CS[id * (NS + 4) + s] = row[c++];
+                    for (int s = 0; s < NS; s++)
+                        F.CMS[id * NS + s] = row[c++];
+                    F.Nchem[id] = (int)row[c++];
+                }
+        iteration = v[15]; // header: NS, grid, block offsets, blocks, rank, size, iteration, num, ...
+        time = restart_time >= 0 ? restart_time : (cfl == 0 ? iteration * dt : 0.0);
+    } catch (...) {
+        std::fclose(f);
+        throw;
+    }
+    std::fclose(f);
+    if (TwoDim.ctx)
+        TwoDim.Upload();
 }
 
 } // namespace eu3dh

@@ -102,6 +102,10 @@ class Flowfield {
     std::vector<double> Mc, Mi, Yi, Di, CMS, CS;
     std::vector<int> Nchem;
     eu3d_ctx *ctx = nullptr;
+    // ghost-fill set (EU3D_BOUNDARY_*) and, for ODW, the inflow state
+    // P rho u v w h gamma Y[NS] T E taken from the initial free stream
+    int boundary = EU3D_BOUNDARY_RSBI;
+    std::vector<double> inflow;
 
     size_t idx(int i, int j, int k) const { return ((size_t)i * NJ + j) * NK + k; }
     eu3d_state HostState();
@@ -110,6 +114,7 @@ class Flowfield {
     void ic_rsbi(const Reaction &react);
     void ic_sbi(const Reaction &react);
     void ic_znd(const Reaction &react);
+    void ic_odw(const Reaction &react);
 };
 
 // TimeAdv (src/TimeAdv.cpp): EE and TVD_RK3 on the device.
@@ -130,6 +135,7 @@ class Euler {
     void Computing(int max_steps);
     void SyncOutput();
     void WriteStateDump(const std::string &path, int num);
+    void ReadStateDump(const std::string &path, double restart_time);
     void SetChemFlags(int f) { chem_flags = f; }
 
     std::string base;

@@ -152,6 +152,13 @@ int eu3d_solver_dump(eu3d_solver *s, const char *path, int num)
     GUARD(s->e->WriteStateDump(path, num));
 }
 
+int eu3d_solver_restart(eu3d_solver *s, const char *path, double time)
+{
+    if (!s || !path)
+        return EU3D_ERR_ARG;
+    GUARD(s->e->ReadStateDump(path, time));
+}
+
 const char *eu3d_solver_error(const eu3d_solver *s) { return s ? s->err.c_str() : "null solver"; }
 
 } // extern "C"

@@ -105,6 +105,17 @@ int eu3d_download_state(eu3d_ctx *ctx, eu3d_state *host);
  * resident_muscl_x/_y/_z + resident_front_thermo (src/FlowfieldCuda.cu:627-696). */
 int eu3d_fill_ghosts(eu3d_ctx *ctx);
 
+/* Ghost-fill set used by eu3d_fill_ghosts.  EU3D_BOUNDARY_RSBI (default) is
+ * FieldBoundary_3dRSBI.  EU3D_BOUNDARY_ODW has no reference counterpart: the
+ * wedge-frame oblique-detonation set (x- and y+ set to the inflow state, x+
+ * mirror copy, y- slip wall v -> -v, z mirror w -> -w; same x -> y -> z order
+ * and extents as the reference's).  inflow: P rho u v w h gamma Y[ns] T E
+ * (ns + 9 values); ignored for RSBI. */
+#define EU3D_BOUNDARY_RSBI 0
+#define EU3D_BOUNDARY_ODW 1
+int eu3d_set_boundary(eu3d_ctx *ctx, int mode, const double *inflow);
+int eu3d_get_boundary(eu3d_ctx *ctx, int *mode);
+
 /* Flowfield::CFLcondition (src/Flowfield.cpp:276-306): dt = cfl * min over the
  * interior of min(dx/(|u|+c), dy/(|v|+c)), clamped to final_time - time. */
 int eu3d_cfl_dt(eu3d_ctx *ctx, double cfl, double time, double final_time, double *dt);

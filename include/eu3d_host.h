@@ -44,6 +44,11 @@ double *eu3d_solver_field(eu3d_solver *s, const char *name);
 int *eu3d_solver_nchem(eu3d_solver *s);
 /* EU3D_STATE_V1_LE dump (src/Euler.cpp:1539-1598) */
 int eu3d_solver_dump(eu3d_solver *s, const char *path, int num);
+/* Restart from such a dump of the same grid and mechanism (no reference
+ * counterpart: the reference writes the dump only).  Restores the interior
+ * state, Nchem, CMS and the iteration counter, and uploads; the time is
+ * `time`, or iteration * dt for a fixed-dt run when time < 0. */
+int eu3d_solver_restart(eu3d_solver *s, const char *path, double time);
 const char *eu3d_solver_error(const eu3d_solver *s);
 
 #ifdef __cplusplus

@@ -773,6 +773,48 @@ static void ic_znd(or_state *s, const or_keyfile *in)
             }
 }
 
+/* Oblique detonation in the wedge frame (new case "ODW"): x runs along the
+ * wedge surface (the wall y = ya), the free stream of H2-air at equivalence
+ * ratio phi, P_pre, T_pre and Mach M_wave meets the wall at wedge_deg:
+ * u = V cos(theta), v = -V sin(theta).  Uniform free stream everywhere; the
+ * oblique shock and the detonation develop from the wall. */
+static void ic_odw(or_state *s, const or_keyfile *in)
+{
+    int ns = s->ns;
+    double phi = kf_double_default(in, "phi", 1.0);
+    double deg = kf_double_default(in, "wedge_deg", 25.0);
+    double theta = deg * 3.14159265358979323846 / 180.0;
+    double mw_u = 2.0 * phi * s->mw[5] + s->mw[4] + 3.76 * s->mw[8];
+    double Y[OR_MAX_NS] = {0}, Cpi[OR_MAX_NS];
+    Y[5] = 2.0 * phi * s->mw[5] / mw_u;
+    Y[4] = s->mw[4] / mw_u;
+    Y[8] = 3.76 * s->mw[8] / mw_u;
+    /* sound speed of the free stream as Flowfield::FieldInitial forms it */
+    double T = s->T_pre, P = s->P_pre;
+    double Wav = 1.0 / sum3(Y, s->mw, ns);
+    double Rgas = s->R * 1000 / Wav;
+    double D = P / (Rgas * T);
+    for (int q = 0; q < ns; q++)
+        Cpi[q] = or_getcpi(T, s->ri[q], q, s->coeff0, s->coeff1, ns);
+    double Cp = sum2(Cpi, Y, ns) * 1000;
+    double G = Cp / (Cp - s->R * sum3(Y, s->mw, ns) * 1000);
+    double V = s->M_wave * sqrt(G * P / D);
+    double u = V * cos(theta), v = -V * sin(theta);
+    for (int i = 0; i < s->NI; i++)
+        for (int j = 0; j < s->NJ; j++)
+            for (int k = 0; k < s->NK; k++) {
+                size_t id = IDX(s, i, j, k);
+                for (int q = 0; q < ns; q++)
+                    s->Yi[id * ns + q] = Y[q];
+                s->U[id] = u;
+                s->V[id] = v;
+                s->W[id] = 0.0;
+                s->P[id] = P;
+                s->T[id] = T;
+            }
+    s->bc_mode = 1;
+}
+
 /* Flowfield::FieldInitial interior tail (src/Flowfield.cpp:227-262) */
 static void field_initial_tail(or_state *s)
 {
@@ -921,11 +963,27 @@ or_state *or_init(const char *ctrl_path, const char *base_dir, char *err, int er
         ic_sbi(s, in);
     } else if (strcmp(s->case_name, "ZND") == 0) {
         ic_znd(s, in);
+    } else if (strcmp(s->case_name, "ODW") == 0) {
+        ic_odw(s, in);
     } else {
         snprintf(err, errlen, "unknown case %s", s->case_name);
         goto fail;
     }
     field_initial_tail(s);
+    if (s->bc_mode == 1) { /* the inflow state is the free stream as the interior holds it */
+        size_t id = IDX(s, s->bc, s->bc, s->bc);
+        s->in_U = s->U[id];
+        s->in_V = s->V[id];
+        s->in_W = s->W[id];
+        s->in_P = s->P[id];
+        s->in_D = s->D[id];
+        s->in_T = s->T[id];
+        s->in_H = s->H[id];
+        s->in_Gamma = s->Gamma[id];
+        s->in_E = s->E[id];
+        for (int q = 0; q < s->ns; q++)
+            s->in_Y[q] = s->Yi[id * s->ns + q];
+    }
     s->dt = 0.0;
     s->time = 0.0;
     free(ctrl);
@@ -988,8 +1046,60 @@ static void copy_ghost(or_state *s, size_t dst, size_t src, int flip_v, int flip
         s->Yi[dst * ns + q] = s->Yi[src * ns + q];
 }
 
+static void set_inflow(or_state *s, size_t d)
+{
+    s->U[d] = s->in_U;
+    s->V[d] = s->in_V;
+    s->W[d] = s->in_W;
+    s->P[d] = s->in_P;
+    s->D[d] = s->in_D;
+    s->T[d] = s->in_T;
+    s->H[d] = s->in_H;
+    s->Gamma[d] = s->in_Gamma;
+    s->E[d] = s->in_E;
+    for (int q = 0; q < s->ns; q++)
+        s->Yi[d * s->ns + q] = s->in_Y[q];
+}
+
+/* Boundary set of the wedge-frame oblique detonation (case "ODW"), in the
+ * order of FieldBoundary_3dRSBI: x- inflow, x+ outflow (mirror copy); over
+ * all i: y- slip wall (mirror, v -> -v), y+ inflow; over all i, j: z mirror
+ * with w -> -w. */
+static void field_boundary_odw(or_state *s)
+{
+    int bc = s->bc, ni = s->ni, nj = s->nj, nk = s->nk;
+    const int xr = 2 * (ni + bc) - 1, yf = 2 * bc - 1, zd = 2 * bc - 1, zu = 2 * (nk + bc) - 1;
+    for (int i = 0; i < bc; i++)
+        for (int j = bc; j < nj + bc; j++)
+            for (int k = bc; k < nk + bc; k++)
+                set_inflow(s, IDX(s, i, j, k));
+    for (int i = ni + bc; i < ni + 2 * bc; i++)
+        for (int j = bc; j < nj + bc; j++)
+            for (int k = bc; k < nk + bc; k++)
+                copy_ghost(s, IDX(s, i, j, k), IDX(s, xr - i, j, k), 0, 0, 1);
+    for (int i = 0; i < ni + 2 * bc; i++)
+        for (int j = 0; j < bc; j++)
+            for (int k = bc; k < nk + bc; k++)
+                copy_ghost(s, IDX(s, i, j, k), IDX(s, i, yf - j, k), 1, 0, 1);
+    for (int i = 0; i < ni + 2 * bc; i++)
+        for (int j = nj + bc; j < nj + 2 * bc; j++)
+            for (int k = bc; k < nk + bc; k++)
+                set_inflow(s, IDX(s, i, j, k));
+    for (int i = 0; i < ni + 2 * bc; i++)
+        for (int j = 0; j < nj + 2 * bc; j++) {
+            for (int k = 0; k < bc; k++)
+                copy_ghost(s, IDX(s, i, j, k), IDX(s, i, j, zd - k), 0, 1, 1);
+            for (int k = nk + bc; k < nk + 2 * bc; k++)
+                copy_ghost(s, IDX(s, i, j, k), IDX(s, i, j, zu - k), 0, 1, 1);
+        }
+}
+
 void or_field_boundary(or_state *s)
 {
+    if (s->bc_mode == 1) {
+        field_boundary_odw(s);
+        return;
+    }
     int bc = s->bc, ni = s->ni, nj = s->nj, nk = s->nk, ns = s->ns;
     /* left: freestream (mirror copy) */
     {

@@ -67,6 +67,12 @@ typedef struct or_state {
 
     double dt, time;
     int iteration;
+
+    /* boundary set: 0 = FieldBoundary_3dRSBI, 1 = wedge-frame oblique
+     * detonation (new case "ODW"): x- and y+ inflow, y- wall, x+ outflow */
+    int bc_mode;
+    double in_U, in_V, in_W, in_P, in_D, in_T, in_H, in_Gamma, in_E; /* inflow state */
+    double in_Y[OR_MAX_NS];
 } or_state;
 
 /* ---- construction / teardown ---------------------------------------- */

@@ -43,6 +43,17 @@ driver_T        3000.0
 """
 
 
+ODW_INPUT = """case            ODW
+P_pre           10132.5
+T_pre           300.0
+M_wave          7.0
+NS              9
+NR              19
+wedge_deg       25.0
+phi             1.0
+"""
+
+
 def write_case(tmpdir, name, ni, nj, nk, extents, input_text, time_adv="EE", dt=3e-8, cfl=0.0,
                final_time=None, steps=1):
     """Write ctrl-<name>.txt (+ grid, input) into tmpdir; return the ctrl path."""
@@ -86,3 +97,11 @@ def znd(tmpdir, ni=201, time_adv="TVD_RK3", cfl=0.5, length=0.01):
     h = length / (ni - 1)
     return write_case(tmpdir, f"znd{ni}", ni, 3, 3, (0.0, length, 0.0, 2 * h, 0.0, 2 * h), ZND_INPUT,
                       time_adv=time_adv, cfl=cfl, dt=0.0)
+
+
+def odw(tmpdir, ni=32, nj=16, nk=4, h=1e-4, time_adv="TVD_RK3", cfl=0.5):
+    """Wedge-frame oblique detonation (ODW: inflow x- and y+, wall y-) on a
+    reduced grid with spacing h: Mach 7 H2-air at 0.1 atm, 300 K, 25 deg."""
+    return write_case(tmpdir, f"odw{ni}x{nj}x{nk}", ni, nj, nk,
+                      (0.0, (ni - 1) * h, 0.0, (nj - 1) * h, 0.0, (nk - 1) * h),
+                      ODW_INPUT, time_adv=time_adv, cfl=cfl, dt=0.0)

@@ -35,10 +35,10 @@ SCALARS = ["U", "V", "W", "P", "D", "T", "C", "Wav", "Rgas", "Cp", "H", "E", "Ga
 SPECIES = ["Mc", "Yi", "Di"]
 
 
-@pytest.mark.parametrize("case", ["rsbi", "sbi", "znd"])
+@pytest.mark.parametrize("case", ["rsbi", "sbi", "znd", "odw"])
 def test_host_initial_state_matches_oracle(tmp_path, case):
     ctrl = {"rsbi": lambda d: cases.rsbi(d), "sbi": lambda d: cases.sbi(d, ni=24, nj=12, nk=12),
-            "znd": lambda d: cases.znd(d, ni=101)}[case](str(tmp_path))
+            "znd": lambda d: cases.znd(d, ni=101), "odw": lambda d: cases.odw(d)}[case](str(tmp_path))
     o = oracle.Oracle(ctrl)
     s = eu3d.Solver(ctrl, device=False)
     assert (s.ni, s.nj, s.nk, s.bc, s.ns) == (o.ni, o.nj, o.nk, o.bc, o.ns)
@@ -103,3 +103,30 @@ def test_generated_mechanism_header_is_current():
     gm = importlib.util.module_from_spec(spec)
     spec.loader.exec_module(gm)
     assert open(gm.OUT).read() == gm.generate()
+
+
+def test_restart_reader_roundtrip(tmp_path):
+    """eu3d_solver_restart restores every dumped field of the interior
+    bit for bit, and rejects a dump of another grid (host only, no GPU)."""
+    ctrl = cases.rsbi(str(tmp_path / "a"), steps=3)
+    a = eu3d.Solver(ctrl, device=False)
+    rng = np.random.default_rng(1)
+    inner = a.interior(a.field("T"))
+    inner += rng.random(inner.shape)                       # a state unlike the initial one
+    a.interior(a.field("CMS"))[...] = rng.random(a.interior(a.field("CMS")).shape)
+    a.interior(a.nchem())[...] = rng.integers(1, 50, a.interior(a.nchem()).shape)
+    pa, pb = str(tmp_path / "a.bin"), str(tmp_path / "b.bin")
+    a.dump(pa)
+    b = eu3d.Solver(ctrl, device=False)
+    b.restart(pa, 1.5e-7)
+    assert b.info()["time"] == 1.5e-7
+    for name in ["D", "T", "U", "V", "W", "E", "P", "H", "Gamma", "C", "Yi", "Di", "CS", "CMS"]:
+        assert np.array_equal(_bits(a.interior(a.field(name))), _bits(b.interior(b.field(name)))), name
+    assert np.array_equal(a.interior(a.nchem()), b.interior(b.nchem()))
+    b.dump(pb)
+    assert open(pa, "rb").read() == open(pb, "rb").read()
+    other = eu3d.Solver(cases.rsbi(str(tmp_path / "c"), ni=19), device=False)
+    with pytest.raises(eu3d.EU3DError):
+        other.restart(pa)
+    with pytest.raises(eu3d.EU3DError):
+        other.restart(str(tmp_path / "missing.bin"))

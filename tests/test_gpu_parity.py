@@ -79,6 +79,49 @@ def test_ghost_fill_index_maps_bitexact(tmp_path):
         assert rel_linf(s.field(n), o.field(n)) <= 1e-14, n
 
 
+def test_ghost_fill_odw_bitexact(tmp_path):
+    """The oblique-detonation boundary set (inflow x- / y+, slip wall y-,
+    mirror x+ and z) on a state with every field varying: bit-exact."""
+    note("ghost fill: odw 32x16x4")
+    ctrl = cases.odw(str(tmp_path))
+    o, s = pair(ctrl)
+    for step in range(5):            # develop the wall shock so the mirrors see structure
+        o.step(o.cfl(0.5, 1.0))
+    s.run(5)
+    s.sync()
+    L = eu3d.load()
+    mode = ctypes.c_int(-1)
+    eu3d.check(L.eu3d_get_boundary(s.ctx, ctypes.byref(mode)), s.ctx)
+    assert mode.value == eu3d.BOUNDARY_ODW
+    # the same interior on both sides, then one ghost fill each
+    for n in ["P", "U", "V", "W", "T", "Yi", "D", "H", "E", "Gamma"]:
+        o.interior(o.field(n))[...] = s.interior(s.field(n))
+    eu3d.check(L.eu3d_fill_ghosts(s.ctx), s.ctx, "fill_ghosts")
+    s.sync()
+    o.field_boundary()
+    for n in ["P", "U", "V", "W", "T", "Yi", "D", "H", "E", "Gamma"]:
+        a, b = s.field(n), o.field(n)
+        assert np.array_equal(a.view(np.int64), b.view(np.int64)), n
+
+
+def test_odw_rk3_cfl(tmp_path):
+    """Oblique detonation set-up, SSP-RK3, CFL 0.5: dt sequence and state."""
+    note("odw rk3 30 steps")
+    ctrl = cases.odw(str(tmp_path))
+    o, s = pair(ctrl)
+    for step in range(30):
+        dt_o = o.cfl(0.5, 1.0)
+        o.step(dt_o)
+        s.run(1)
+        dt_s = s.info()["last_dt"]
+        assert abs(dt_s - dt_o) <= 1e-12 * dt_o, (step, dt_s, dt_o)
+        if step == 0:
+            s.sync()
+            compare_state(o, s, 1e-12, "ODW RK3 1 step")
+    s.sync()
+    compare_state(o, s, 1e-8, "ODW RK3 30 steps")
+
+
 def test_thermo_probe(tmp_path):
     """NASA-9 cp/h/s at the reference's probe temperatures (src/Reaction.cpp:333-335) and a sweep."""
     o, s = pair(cases.rsbi(str(tmp_path)))
@@ -218,6 +261,30 @@ def test_state_dump_v1(tmp_path):
     s.dump(pa)
     assert o.write_dump(pb) == 0
     assert compare_states.compare(pb, [pa]) == 0
+
+
+@pytest.mark.parametrize("case", ["rsbi", "sbi"])
+def test_restart_continues_bitwise(tmp_path, case):
+    """N steps, or N/2 steps -> dump -> restart in a new solver -> N/2 steps:
+    the same bits (fixed-dt EE RSBI with stiff chemistry; CFL RK3 SBI)."""
+    if case == "rsbi":
+        ctrl = cases.rsbi(str(tmp_path), steps=6, T_pre=1200.0, dt=3e-7)
+    else:
+        ctrl = cases.sbi(str(tmp_path), ni=24, nj=12, nk=12)
+    a, b = eu3d.Solver(ctrl), eu3d.Solver(ctrl)
+    a.run(6)
+    b.run(3)
+    mid = str(tmp_path / "mid.bin")
+    b.dump(mid)
+    c = eu3d.Solver(ctrl)
+    c.restart(mid, b.info()["time"])
+    c.run(3)
+    pa, pc = str(tmp_path / "a.bin"), str(tmp_path / "c.bin")
+    a.dump(pa)
+    c.dump(pc)
+    assert a.info()["iteration"] == c.info()["iteration"] == 6
+    assert a.info()["time"] == c.info()["time"]
+    assert open(pa, "rb").read() == open(pc, "rb").read()
 
 
 def test_generated_mechanism_equals_runtime_tables(tmp_path):
