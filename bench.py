@@ -109,6 +109,105 @@ def cpu_baseline(ctrl, base, threads):
                       f"oracle/liboracle_omp.so with {used} OpenMP threads, {el:.2f} s"}
 
 
+def ignition_length(s, T_ign=2500.0):
+    """Mean over z of the first x (cells from the inflow) where T > T_ign in
+    the row next to the wall: where the oblique detonation meets the wall."""
+    T = s.interior(s.field("T"))[:, 0, :]
+    hot = T > T_ign
+    first = np.where(hot.any(axis=0), hot.argmax(axis=0), T.shape[0])
+    return float(first.mean())
+
+
+def odw_side_by_side(args):
+    """Wedge-frame oblique detonation (input/ctrl-3dODW.txt, 512x256x64, or
+    ctrl-3dODW-large.txt): develop the detonation, dump, time K steps with
+    the FP64 chemistry, restart from the dump and time K steps with the
+    surrogate; compare the two end states."""
+    import importlib
+    import tempfile
+    eu3d = eu3d_pkg()
+    sur = importlib.import_module("eu3d-gpu_amd.surrogate")
+    L = eu3d.load()
+    base = os.path.join(REPO, "eu3d-gpu_amd")
+    ctrl = os.path.join(base, "input", "ctrl-3dODW-large.txt" if args.workload == "odw-large" else "ctrl-3dODW.txt")
+    s = eu3d.Solver(ctrl, base_dir=base)
+    n_int = s.ni * s.nj * s.nk
+    flags = eu3d.CHEM_COMPACT | eu3d.CHEM_KEEP_CMS
+    done = 0
+    while done < args.develop:
+        k = min(500, args.develop - done)
+        s.run(k, flags)
+        eu3d.check(L.eu3d_synchronize(s.ctx), s.ctx)
+        done += k
+        rc, h = s.health()
+        print(f"bench: odw developed {done} steps, t = {s.info()['time']:.3e} s, max Nchem {h.max_nchem}, "
+              f"stiff cells {h.stiff_cells}", file=sys.stderr, flush=True)
+    t_dev = s.info()["time"]
+    fd, dump = tempfile.mkstemp(suffix=".bin")
+    os.close(fd)
+    s.dump(dump)
+    res = {}
+    names = sur.species_names(os.path.join(base, "input", "H2_JA919.txt"))
+    net = None
+    for mode in ("fp64", "surrogate"):
+        if mode == "surrogate":
+            s.restart(dump, t_dev)
+            net = sur.Network.load_file(args.model or sur.DEFAULT_MODEL, max_batch=65536)
+            sur.attach(s.ctx, net, names, min_nchem=args.min_nchem)
+        f = flags | (eu3d.CHEM_SURROGATE if mode == "surrogate" else 0)
+        eu3d.check(L.eu3d_synchronize(s.ctx), s.ctx)
+        routed = fallback = 0
+        failed = None
+        t0 = time.perf_counter()
+        for step in range(args.steps):
+            try:
+                s.run(1, f)
+            except eu3d.EU3DError as e:
+                failed = f"step {step}: {e}"
+                print(f"bench: odw {mode} run stopped at {failed}", file=sys.stderr, flush=True)
+                break
+            if mode == "surrogate":
+                st = sur.stats(s.ctx)
+                routed += st["routed"]
+                fallback += st["fallback"]
+        eu3d.check(L.eu3d_synchronize(s.ctx), s.ctx)
+        el = time.perf_counter() - t0
+        if failed:
+            res[mode] = dict(failed=failed, routed=routed, fallback=fallback)
+            continue
+        rc, h = s.health()
+        s.sync()
+        res[mode] = dict(ms_per_step=el / args.steps * 1e3, value=n_int * args.steps / el, health_rc=rc,
+                         T=np.array(s.interior(s.field("T"))), ign=ignition_length(s), routed=routed,
+                         fallback=fallback, stiff_last=h.stiff_cells, max_nchem_last=h.max_nchem)
+    os.remove(dump)
+    a, b = res["fp64"], res["surrogate"]
+    if "failed" in a:
+        raise RuntimeError(f"odw FP64 run failed: {a['failed']}")
+    out = {
+        "metric": METRIC, "value": a["value"], "unit": "cell-steps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.develop, "ms_per_step": a["ms_per_step"], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"oblique detonation, wedge frame {s.ni}x{s.nj}x{s.nk}, Mach 8 H2-air 1 atm 300 K, "
+                               f"25 deg, timed from a developed front ({args.develop} steps)",
+                   "grid": [s.ni, s.nj, s.nk]},
+        "fp64": {"ms_per_step": a["ms_per_step"], "stiff_cells_last": a["stiff_last"],
+                 "max_nchem_last": a["max_nchem_last"]},
+        "surrogate": b if "failed" in b else {"min_nchem": args.min_nchem, "ms_per_step": b["ms_per_step"],
+                                              "value": b["value"],
+                      "speedup_end_to_end": a["ms_per_step"] / b["ms_per_step"], "routed": b["routed"],
+                      "fallback": b["fallback"], "health_rc": b["health_rc"],
+                      "peakT_fp64": float(a["T"].max()), "peakT_surrogate": float(b["T"].max()),
+                      "peakT_rel_err": float(abs(b["T"].max() - a["T"].max()) / a["T"].max()),
+                      "T_linf": float(np.abs(a["T"] - b["T"]).max()),
+                      "wall_ignition_cells_fp64": a["ign"], "wall_ignition_cells_surrogate": b["ign"]},
+    }
+    s.close()
+    if net is not None:
+        net.close()
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -119,6 +218,11 @@ def main():
     ap.add_argument("--no-surrogate", action="store_true",
                     help="skip the chemistry-call comparison FP64 vs the committed surrogate")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--workload", choices=["sbi", "odw", "odw-large"], default="sbi",
+                    help="sbi: the default line; odw: oblique detonation, FP64 and surrogate side by side")
+    ap.add_argument("--develop", type=int, default=3000, help="odw: steps before the timed region")
+    ap.add_argument("--model", default=None, help="odw: surrogate model file (default: the committed one)")
+    ap.add_argument("--min-nchem", type=int, default=2, help="odw: cells with Nchem >= this use the surrogate")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device once (torch) and exit: the known-size kernel that "
                          "scales the FETCH_SIZE/WRITE_SIZE counters")
@@ -139,6 +243,9 @@ def main():
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(multi-GPU decomposition is out of scope)", file=sys.stderr)
         sys.exit(2)
+    if args.workload != "sbi":
+        odw_side_by_side(args)
+        return
 
     eu3d = eu3d_pkg()
     L = eu3d.load()

@@ -820,6 +820,23 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(EU3
     }
 }
 
+// resident_health_kernel (src/ReactionCuda.cu:2265-2273): cells whose density
+// or temperature is not finite after the chemistry.
+__global__ void __launch_bounds__(kBlock)
+    k_chem_health(Grid g, const double *__restrict__ prim, const double *__restrict__ T, ChemCounters *cnt)
+{
+    long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    int i, j, k;
+    bool bad = false;
+    if (interior_cell(g, t, i, j, k)) {
+        const long long c = gidx(g, i, j, k);
+        bad = !isfinite(prim[g.N + c]) || !isfinite(T[c]);
+    }
+    unsigned long long m = __ballot(bad);
+    if ((threadIdx.x & 63) == 0 && m)
+        atomicAdd(&cnt->nonfinite, (unsigned long long)__popcll(m));
+}
+
 // Remaining sub-steps 1..Nchem-1 of one cell.
 template <int NS, class MT>
 __device__ __forceinline__ void chem_rest(const Grid &g, const MT &m, long long c, int n, double dtm,
@@ -1315,6 +1332,7 @@ static void launch_chem(eu3d_ctx *ctx, double dt, int flags)
                                                                            ctx->T, ctx->E, ctx->C, ctx->CMS,
                                                                            ctx->nchem, keep);
     }
+    k_chem_health<<<blocks_for(ctx->n_int), kBlock, 0, ctx->stream>>>(g, ctx->prim, ctx->T, ctx->cnt);
 }
 
 // Chemistry with the surrogate: cells with Nchem >= sur_min go through the
@@ -1357,6 +1375,7 @@ static void launch_chem_sur(eu3d_ctx *ctx, double dt, int flags, int *rc)
     }
     k_chem_queue<NS, MT, true><<<ctx->ncu * 4, kBlock, 0, ctx->stream>>>(
         g, ctx->mech, dt, ctx->CS, ctx->prim, ctx->T, ctx->E, ctx->C, ctx->CMS, ctx->nchem, ctx->queue, ctx->cnt, keep);
+    k_chem_health<<<blocks_for(ctx->n_int), kBlock, 0, ctx->stream>>>(g, ctx->prim, ctx->T, ctx->cnt);
 }
 
 template <int NS, class MT>

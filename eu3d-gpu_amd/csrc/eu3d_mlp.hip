@@ -106,6 +106,81 @@ __global__ void __launch_bounds__(256)
     }
 }
 
+// C[M x N] = A[M x K] . Bt[N x K]^T with both operands contiguous in k
+// (row strides lda, ldb multiples of 8): every lane fetches its 8 k-values
+// of a 32-row fragment with one 16-byte load.  A wave computes WM x WN tiles
+// of 32 x 32 (fragments reused across the tile), a block is 4 waves stacked
+// in m: (4 WM 32) x (WN 32) outputs.  M multiple of 32 WM, N of 32 WN, K of 16.
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+template <int EPI, int WM, int WN>
+__global__ void __launch_bounds__(256)
+    k_gemm_nt(int M, int N, int K, const bf16 *__restrict__ A, int lda, const bf16 *__restrict__ Bt, int ldb,
+              const float *__restrict__ bias, float *__restrict__ Cf, bf16 *__restrict__ Cb, int ldc,
+              const float *__restrict__ Zaux)
+{
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    const int m0 = (blockIdx.x * 4 + w) * (32 * WM);
+    const int n0 = blockIdx.y * (32 * WN);
+    if (m0 >= M || n0 >= N)
+        return;
+    const int r = l & 31, q = l >> 5;
+    f32x16 acc[WM][WN];
+#pragma unroll
+    for (int a = 0; a < WM; a++)
+#pragma unroll
+        for (int b = 0; b < WN; b++)
+#pragma unroll
+            for (int v = 0; v < 16; v++)
+                acc[a][b][v] = 0.f;
+    const bf16 *Ap[WM], *Bp[WN];
+#pragma unroll
+    for (int a = 0; a < WM; a++)
+        Ap[a] = A + (long long)(m0 + 32 * a + r) * lda + 8 * q;
+#pragma unroll
+    for (int b = 0; b < WN; b++)
+        Bp[b] = Bt + (long long)(n0 + 32 * b + r) * ldb + 8 * q;
+    for (int k0 = 0; k0 < K; k0 += 16) {
+        s16x8 af[WM], bfr[WN];
+#pragma unroll
+        for (int a = 0; a < WM; a++) {
+            u32x4 v = *reinterpret_cast<const u32x4 *>(Ap[a] + k0);
+            af[a] = __builtin_bit_cast(s16x8, v);
+        }
+#pragma unroll
+        for (int b = 0; b < WN; b++) {
+            u32x4 v = *reinterpret_cast<const u32x4 *>(Bp[b] + k0);
+            bfr[b] = __builtin_bit_cast(s16x8, v);
+        }
+#pragma unroll
+        for (int a = 0; a < WM; a++)
+#pragma unroll
+            for (int b = 0; b < WN; b++)
+                acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(af[a], bfr[b], acc[a][b], 0, 0, 0);
+    }
+#pragma unroll
+    for (int a = 0; a < WM; a++)
+#pragma unroll
+        for (int b = 0; b < WN; b++)
+#pragma unroll
+            for (int v = 0; v < 16; v++) {
+                const int m = m0 + 32 * a + 8 * (v >> 2) + 4 * q + (v & 3);
+                const int n = n0 + 32 * b + r;
+                const long long o = (long long)m * ldc + n;
+                float x = acc[a][b][v];
+                if (EPI == EPI_BIAS_GELU) {
+                    float z = x + bias[n];
+                    Cf[o] = z;
+                    Cb[o] = f2bf(gelu(z));
+                } else if (EPI == EPI_GELU_ONLY) {
+                    Cb[o] = f2bf(gelu(x + bias[n]));
+                } else if (EPI == EPI_BIAS) {
+                    Cf[o] = x + bias[n];
+                } else if (EPI == EPI_GELU_BWD) {
+                    Cb[o] = f2bf(x * gelu_grad(Zaux[o]));
+                }
+            }
+}
+
 // features (raw, n_in per row, gathered through idx) -> normalized bf16 [B x kIn];
 // targets -> normalized fp32 [B x kOut] (padding zero)
 __global__ void k_prep(int B, int n_in, int n_out, const long long *__restrict__ idx, const float *__restrict__ x,
@@ -159,9 +234,11 @@ __global__ void k_colsum(int B, int N, const bf16 *__restrict__ dZ, float *__res
 }
 
 // Adam on fp32 master parameters; refreshes the bf16 copy
+// Adam on fp32 master parameters; refreshes the bf16 copy and, for a weight
+// matrix [rows x cols] (cols > 0), its transposed bf16 copy.
 __global__ void k_adam(long long n, float *__restrict__ p, float *__restrict__ g, float *__restrict__ m,
-                       float *__restrict__ v, bf16 *__restrict__ pb, float lr, float b1, float b2, float eps,
-                       float c1, float c2)
+                       float *__restrict__ v, bf16 *__restrict__ pb, bf16 *__restrict__ pt, int rows, int cols,
+                       float lr, float b1, float b2, float eps, float c1, float c2)
 {
     long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
     if (i >= n)
@@ -173,16 +250,24 @@ __global__ void k_adam(long long n, float *__restrict__ p, float *__restrict__ g
     v[i] = vi;
     float pi = p[i] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
     p[i] = pi;
-    if (pb)
-        pb[i] = f2bf(pi);
+    if (pb) {
+        bf16 h = f2bf(pi);
+        pb[i] = h;
+        pt[(i % cols) * rows + i / cols] = h;
+    }
     g[i] = 0.f;
 }
 
-__global__ void k_to_bf16(long long n, const float *__restrict__ a, bf16 *__restrict__ b)
+// bf16 copy of a [rows x cols] matrix and its transpose
+__global__ void k_to_bf16(long long n, const float *__restrict__ a, bf16 *__restrict__ b, bf16 *__restrict__ bt,
+                          int rows, int cols)
 {
     long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-    if (i < n)
-        b[i] = f2bf(a[i]);
+    if (i < n) {
+        bf16 h = f2bf(a[i]);
+        b[i] = h;
+        bt[(i % cols) * rows + i / cols] = h;
+    }
 }
 
 unsigned int nblk(long long n, int b = 256) { return (unsigned int)((n + b - 1) / b); }
@@ -195,7 +280,7 @@ struct eu3d_mlp {
     hipStream_t stream;
     float *W[kLayers], *b[kLayers], *gW[kLayers], *gb[kLayers], *mW[kLayers], *vW[kLayers], *mb[kLayers],
         *vb[kLayers];
-    bf16 *Wb[kLayers];
+    bf16 *Wb[kLayers], *Wt[kLayers]; // bf16 weights [K x N] and transposed [N x K]
     float *norm; // in mean[kIn], in std[kIn], out mean[kOut], out std[kOut]
     bf16 *X0, *Hb[kLayers - 1], *dZ[kLayers];
     float *Z[kLayers - 1], *Y, *T0;
@@ -220,20 +305,19 @@ struct eu3d_mlp {
 static int mlp_forward(eu3d_mlp *mlp, hipStream_t st, int B, bool keep_z)
 {
     const int H = mlp->H;
-    dim3 blk(256), grid_h(nblk(B, 64), nblk(H, 64), 1);
+    const dim3 blk(256), grid_h((B + 255) / 256, H / 64), grid_o((B + 255) / 256, 1);
     for (int l = 0; l < kLayers - 1; l++) {
         const bf16 *in = l == 0 ? mlp->X0 : mlp->Hb[l - 1];
         const int K = mlp->K[l];
         if (keep_z)
-            k_gemm<EPI_BIAS_GELU><<<grid_h, blk, 0, st>>>(B, H, K, in, K, 1, mlp->Wb[l], H, 1, mlp->b[l], mlp->Z[l],
-                                                         mlp->Hb[l], H, nullptr);
+            k_gemm_nt<EPI_BIAS_GELU, 2, 2><<<grid_h, blk, 0, st>>>(B, H, K, in, K, mlp->Wt[l], K, mlp->b[l],
+                                                                   mlp->Z[l], mlp->Hb[l], H, nullptr);
         else
-            k_gemm<EPI_GELU_ONLY><<<grid_h, blk, 0, st>>>(B, H, K, in, K, 1, mlp->Wb[l], H, 1, mlp->b[l], nullptr,
-                                                         mlp->Hb[l], H, nullptr);
+            k_gemm_nt<EPI_GELU_ONLY, 2, 2><<<grid_h, blk, 0, st>>>(B, H, K, in, K, mlp->Wt[l], K, mlp->b[l], nullptr,
+                                                                   mlp->Hb[l], H, nullptr);
     }
-    k_gemm<EPI_BIAS><<<dim3(nblk(B, 64), 1, 1), blk, 0, st>>>(B, kOut, H, mlp->Hb[kLayers - 2], H, 1,
-                                                             mlp->Wb[kLayers - 1], kOut, 1, mlp->b[kLayers - 1],
-                                                             mlp->Y, nullptr, kOut, nullptr);
+    k_gemm_nt<EPI_BIAS, 2, 1><<<grid_o, blk, 0, st>>>(B, kOut, H, mlp->Hb[kLayers - 2], H, mlp->Wt[kLayers - 1], H,
+                                                      mlp->b[kLayers - 1], mlp->Y, nullptr, kOut, nullptr);
     MCK(hipGetLastError());
     return EU3D_OK;
 }
@@ -251,9 +335,9 @@ static int mlp_backward(eu3d_mlp *mlp, int B)
         k_gemm<EPI_ATOMIC><<<dim3(nblk(K, 64), nblk(N, 64), split), blk, 0, mlp->stream>>>(
             K, N, B, Ain[l], 1, K, mlp->dZ[l], N, 1, nullptr, mlp->gW[l], nullptr, N, nullptr);
         k_colsum<<<dim3(64, nblk(N, 64)), 64, 0, mlp->stream>>>(B, N, mlp->dZ[l], mlp->gb[l]);
-        if (l > 0) // dZ_{l-1} = (dZ_l . W_l^T) * gelu'(Z_{l-1})
-            k_gemm<EPI_GELU_BWD><<<dim3(nblk(B, 64), nblk(K, 64), 1), blk, 0, mlp->stream>>>(
-                B, K, N, mlp->dZ[l], N, 1, mlp->Wb[l], 1, N, nullptr, nullptr, mlp->dZ[l - 1], K, mlp->Z[l - 1]);
+        if (l > 0) // dZ_{l-1} = (dZ_l . W_l^T) * gelu'(Z_{l-1}); W_l [K x N] is the k-contiguous operand
+            k_gemm_nt<EPI_GELU_BWD, 2, 2><<<dim3((B + 255) / 256, K / 64), blk, 0, mlp->stream>>>(
+                B, K, N, mlp->dZ[l], N, mlp->Wb[l], N, nullptr, nullptr, mlp->dZ[l - 1], K, mlp->Z[l - 1]);
     }
     MCK(hipGetLastError());
     return EU3D_OK;
@@ -271,7 +355,7 @@ int eu3d_mlp_destroy(eu3d_mlp *mlp)
         (void)hipStreamSynchronize(mlp->stream);
     for (int l = 0; l < kLayers; l++) {
         void *p[] = {mlp->W[l], mlp->b[l], mlp->gW[l], mlp->gb[l], mlp->mW[l], mlp->vW[l], mlp->mb[l], mlp->vb[l],
-                     mlp->Wb[l], mlp->dZ[l]};
+                     mlp->Wb[l], mlp->Wt[l], mlp->dZ[l]};
         for (void *q : p)
             if (q)
                 (void)hipFree(q);
@@ -333,6 +417,7 @@ int eu3d_mlp_create(int n_in, int hidden, int n_out, int max_batch, unsigned lon
         MCK(hipMalloc(&mlp->mW[l], nw * 4));
         MCK(hipMalloc(&mlp->vW[l], nw * 4));
         MCK(hipMalloc(&mlp->Wb[l], nw * 2));
+        MCK(hipMalloc(&mlp->Wt[l], nw * 2));
         MCK(hipMalloc(&mlp->b[l], N[l] * 4));
         MCK(hipMalloc(&mlp->gb[l], N[l] * 4));
         MCK(hipMalloc(&mlp->mb[l], N[l] * 4));
@@ -345,7 +430,7 @@ int eu3d_mlp_create(int n_in, int hidden, int n_out, int max_batch, unsigned lon
         MCK(hipMemset(mlp->gb[l], 0, N[l] * 4));
         MCK(hipMemset(mlp->mb[l], 0, N[l] * 4));
         MCK(hipMemset(mlp->vb[l], 0, N[l] * 4));
-        k_to_bf16<<<nblk(nw), 256, 0, mlp->stream>>>(nw, mlp->W[l], mlp->Wb[l]);
+        k_to_bf16<<<nblk(nw), 256, 0, mlp->stream>>>(nw, mlp->W[l], mlp->Wb[l], mlp->Wt[l], K[l], N[l]);
         MCK(hipMalloc(&mlp->dZ[l], (size_t)max_batch * N[l] * 2));
     }
     for (int l = 0; l < kLayers - 1; l++) {
@@ -441,7 +526,7 @@ static int params_io(eu3d_mlp *mlp, float *host, int to_host, int which /*0 para
         } else {
             MCK(hipMemcpy(W, host + o, nw * 4, hipMemcpyHostToDevice));
             MCK(hipMemcpy(b, host + o + nw, mlp->N[l] * 4, hipMemcpyHostToDevice));
-            k_to_bf16<<<nblk(nw), 256, 0, mlp->stream>>>(nw, W, mlp->Wb[l]);
+            k_to_bf16<<<nblk(nw), 256, 0, mlp->stream>>>(nw, W, mlp->Wb[l], mlp->Wt[l], mlp->K[l], mlp->N[l]);
         }
         o += nw + mlp->N[l];
     }
@@ -596,9 +681,11 @@ int eu3d_mlp_train(eu3d_mlp *mlp, int epochs, int B, float lr, float lr_end, uns
             for (int l = 0; l < kLayers; l++) {
                 long long nw = (long long)mlp->K[l] * mlp->N[l];
                 k_adam<<<nblk(nw), 256, 0, mlp->stream>>>(nw, mlp->W[l], mlp->gW[l], mlp->mW[l], mlp->vW[l],
-                                                          mlp->Wb[l], lr_s, b1, b2, eps, c1, c2);
+                                                          mlp->Wb[l], mlp->Wt[l], mlp->K[l], mlp->N[l], lr_s, b1,
+                                                          b2, eps, c1, c2);
                 k_adam<<<nblk(mlp->N[l]), 256, 0, mlp->stream>>>(mlp->N[l], mlp->b[l], mlp->gb[l], mlp->mb[l],
-                                                                 mlp->vb[l], nullptr, lr_s, b1, b2, eps, c1, c2);
+                                                                 mlp->vb[l], nullptr, nullptr, 0, 0, lr_s, b1, b2,
+                                                                 eps, c1, c2);
             }
         }
         double L = 0.0;

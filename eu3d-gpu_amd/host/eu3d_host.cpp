@@ -829,6 +829,12 @@ void Euler::Computing(int max_steps)
         time += TwoDim.dt;
         iteration += 1;
         n++;
+        // the reference's per-step resident health check and abort
+        // (src/ReactionCuda.cu:2460-2465): invalid Nchem or a non-finite cell
+        if (!(chem_flags & EU3D_CHEM_SKIP)) {
+            eu3d_health_info h;
+            check(eu3d_health(TwoDim.ctx, &h), TwoDim.ctx, "eu3d_health");
+        }
     }
 }
 
