@@ -43,6 +43,11 @@ struct ChemCounters {
     unsigned int pad1;
     unsigned int pad2;
     unsigned long long fail_hull, fail_elem, fail_neg, fail_T;
+    // stiff-queue sort: bucket sizes and fill cursors (bucket = floor(log2 Nchem))
+    unsigned int bucket_n[32];
+    unsigned int bucket_cur[32];
+    // invalid or non-finite cells since the last eu3d_health (not reset per call)
+    unsigned long long sticky_bad;
 };
 
 // Acceptance parameters of the surrogate (eu3d_surrogate_params) on the device.
@@ -70,7 +75,7 @@ struct eu3d_ctx {
     DevMech *mech;
     double *x, *y, *z;
     double *prim, *T, *E, *C, *CS, *CS0, *CMS, *scratch;
-    int *nchem, *queue;
+    int *nchem, *queue, *queue2; // queue2: the stiff queue sorted by Nchem
     ChemCounters *cnt;
     // learned surrogate (not owned) and its work lists
     eu3d_mlp *mlp;
@@ -613,6 +618,10 @@ __global__ void k_chem_reset(ChemCounters *c)
     c->fail_elem = 0;
     c->fail_neg = 0;
     c->fail_T = 0;
+    for (int b = 0; b < 32; b++) {
+        c->bucket_n[b] = 0;
+        c->bucket_cur[b] = 0;
+    }
 }
 
 template <int NS>
@@ -778,10 +787,12 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(EU3
             sur = true;
         }
         nchem[c] = n;
-        if (n >= 1)
+        if (n >= 1) {
             stiff = !sur && n > 1;
-        else
+        } else {
             atomicAdd(&cnt->invalid, 1ull);
+            atomicAdd(&cnt->sticky_bad, 1ull);
+        }
     }
     if (SUR) {
         unsigned long long smask = __ballot(sur);
@@ -833,8 +844,10 @@ __global__ void __launch_bounds__(kBlock)
         bad = !isfinite(prim[g.N + c]) || !isfinite(T[c]);
     }
     unsigned long long m = __ballot(bad);
-    if ((threadIdx.x & 63) == 0 && m)
+    if ((threadIdx.x & 63) == 0 && m) {
         atomicAdd(&cnt->nonfinite, (unsigned long long)__popcll(m));
+        atomicAdd(&cnt->sticky_bad, (unsigned long long)__popcll(m));
+    }
 }
 
 // Remaining sub-steps 1..Nchem-1 of one cell.
@@ -847,7 +860,71 @@ __device__ __forceinline__ void chem_rest(const Grid &g, const MT &m, long long 
         chem_step_cell<NS, false>(g, m, c, dtm, n, CS, prim, T, E, C, CMS, keep_cms);
 }
 
-// Compacted: waves fetch 64 queued cells at a time until the queue is drained.
+// Stiff-queue order: cells sorted into buckets of floor(log2 Nchem), the
+// buckets in descending order, so that the longest cells start first and the
+// cells a wave holds at a time need similar sub-step counts.  The order does
+// not change any result (cells are independent).
+constexpr int kSortChunk = 4096; // queue entries per sorting block
+
+__device__ __forceinline__ int nchem_bucket(int n) { return 31 - __clz(max(n, 1)); }
+
+__device__ __forceinline__ long long queue_cell(int e) { return e < 0 ? ~(long long)e : (long long)e; }
+
+// pass 1: bucket sizes (block histogram in LDS, one global add per bucket)
+__global__ void __launch_bounds__(kBlock)
+    k_queue_hist(const int *__restrict__ queue, const int *__restrict__ nchem, ChemCounters *cnt)
+{
+    __shared__ unsigned int h[32];
+    const unsigned int total = cnt->queue_count;
+    const unsigned int q0 = blockIdx.x * kSortChunk;
+    if (q0 >= total)
+        return;
+    const unsigned int q1 = min(total, q0 + kSortChunk);
+    if (threadIdx.x < 32)
+        h[threadIdx.x] = 0;
+    __syncthreads();
+    for (unsigned int q = q0 + threadIdx.x; q < q1; q += kBlock)
+        atomicAdd(&h[nchem_bucket(nchem[queue_cell(queue[q])])], 1u);
+    __syncthreads();
+    if (threadIdx.x < 32 && h[threadIdx.x])
+        atomicAdd(&cnt->bucket_n[threadIdx.x], h[threadIdx.x]);
+}
+
+// pass 2: scatter into the sorted queue (a range per bucket and block)
+__global__ void __launch_bounds__(kBlock) k_queue_scatter(const int *__restrict__ queue, const int *__restrict__ nchem,
+                                                          int *__restrict__ out, ChemCounters *cnt)
+{
+    __shared__ unsigned int h[32], base[32];
+    const unsigned int total = cnt->queue_count;
+    const unsigned int q0 = blockIdx.x * kSortChunk;
+    if (q0 >= total)
+        return;
+    const unsigned int q1 = min(total, q0 + kSortChunk);
+    if (threadIdx.x < 32)
+        h[threadIdx.x] = 0;
+    __syncthreads();
+    for (unsigned int q = q0 + threadIdx.x; q < q1; q += kBlock)
+        atomicAdd(&h[nchem_bucket(nchem[queue_cell(queue[q])])], 1u);
+    __syncthreads();
+    if (threadIdx.x < 32) {
+        const int b = threadIdx.x;
+        unsigned int off = 0; // buckets above b come first
+        for (int u = 31; u > b; u--)
+            off += cnt->bucket_n[u];
+        base[b] = h[b] ? off + atomicAdd(&cnt->bucket_cur[b], h[b]) : 0;
+        h[b] = 0;
+    }
+    __syncthreads();
+    for (unsigned int q = q0 + threadIdx.x; q < q1; q += kBlock) {
+        const int e = queue[q];
+        const int b = nchem_bucket(nchem[queue_cell(e)]);
+        out[base[b] + atomicAdd(&h[b], 1u)] = e;
+    }
+}
+
+// Compacted stiff cells: every lane takes the next queued cell as soon as
+// its current one is done (wave-aggregated fetch), so lanes do not idle while
+// others finish longer cells.  Each sub-step runs from and to memory.
 // FB: the queue also holds cells the surrogate rejected, as ~c; those run
 // every sub-step from their unchanged state.
 template <int NS, class MT, bool FB = false>
@@ -858,25 +935,57 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(EU3
                  ChemCounters *cnt, int keep_cms)
 {
     const unsigned int total = cnt->queue_count;
-    const int lane = threadIdx.x & 63;
+    const auto &m = mech_ref<MT>(mp);
+    long long c = -1;
+    int n = 0, step = 0; // the cell is done when step >= n
+    bool fresh = false;  // FB: sub-step 0 (with the Nchem prediction) still to do
+    bool more = true;    // wave-uniform: the queue may still hold cells
     for (;;) {
-        unsigned int base = 0;
-        if (lane == 0)
-            base = atomicAdd(&cnt->fetch, 64u);
-        base = __shfl(base, 0, 64);
-        if (base >= total)
+        const bool need = step >= n;
+        if (more) {
+            const unsigned long long want = __ballot(need);
+            if (want) {
+                const int leader = __ffsll((long long)want) - 1;
+                unsigned int base = 0;
+                if ((int)(threadIdx.x & 63) == leader)
+                    base = atomicAdd(&cnt->fetch, (unsigned int)__popcll(want));
+                base = __shfl(base, leader, 64);
+                if (base + (unsigned int)__popcll(want) >= total)
+                    more = false;
+                if (need) {
+                    const unsigned int q = base + (unsigned int)__popcll(want & lanemask_lt());
+                    if (q < total) {
+                        const int e = queue[q];
+                        if (FB && e < 0) {
+                            c = ~(long long)e;
+                            fresh = true;
+                            n = 1;
+                            step = 0;
+                        } else {
+                            c = e;
+                            fresh = false;
+                            n = nchem[c];
+                            step = 1;
+                        }
+                    } else {
+                        c = -1;
+                        n = 0;
+                        step = 0;
+                    }
+                }
+            }
+        }
+        const bool active = c >= 0 && step < n;
+        if (!__any(active))
             break;
-        unsigned int q = base + lane;
-        if (q < total) {
-            long long c = queue[q];
-            if (FB && c < 0) {
-                c = ~c;
-                const auto &m = mech_ref<MT>(mp);
-                int n = chem_step_cell<NS, true>(g, m, c, dtm, 0, CS, prim, T, E, C, CMS, keep_cms);
-                if (n > 1)
-                    chem_rest<NS>(g, m, c, n, dtm, CS, prim, T, E, C, CMS, keep_cms);
+        if (active) {
+            if (FB && fresh) {
+                n = chem_step_cell<NS, true>(g, m, c, dtm, 0, CS, prim, T, E, C, CMS, keep_cms);
+                fresh = false;
+                step = 1;
             } else {
-                chem_rest<NS>(g, mech_ref<MT>(mp), c, nchem[c], dtm, CS, prim, T, E, C, CMS, keep_cms);
+                chem_step_cell<NS, false>(g, m, c, dtm, n, CS, prim, T, E, C, CMS, keep_cms);
+                step++;
             }
         }
     }
@@ -1313,6 +1422,14 @@ static void launch_ghost_y(eu3d_ctx *ctx)
     k_ghost_y<NS, MT><<<blocks_for(n), kBlock, 0, ctx->stream>>>(ctx->g, ctx->mech, ctx->prim, ctx->T, ctx->E);
 }
 
+// queue -> queue2 in descending Nchem buckets
+static void sort_queue(eu3d_ctx *ctx)
+{
+    const unsigned int nb = blocks_for(ctx->n_int, kSortChunk);
+    k_queue_hist<<<nb, kBlock, 0, ctx->stream>>>(ctx->queue, ctx->nchem, ctx->cnt);
+    k_queue_scatter<<<nb, kBlock, 0, ctx->stream>>>(ctx->queue, ctx->nchem, ctx->queue2, ctx->cnt);
+}
+
 template <int NS, class MT>
 static void launch_chem(eu3d_ctx *ctx, double dt, int flags)
 {
@@ -1324,9 +1441,10 @@ static void launch_chem(eu3d_ctx *ctx, double dt, int flags)
         g, ctx->mech, dt, ctx->CS, ctx->prim, ctx->T, ctx->E, ctx->C, ctx->CMS, ctx->nchem, ctx->queue, ctx->cnt,
         compact, keep, 0, nullptr);
     if (compact) {
-        k_chem_queue<NS, MT><<<ctx->ncu * 4, kBlock, 0, ctx->stream>>>(g, ctx->mech, dt, ctx->CS, ctx->prim, ctx->T, ctx->E,
-                                                              ctx->C, ctx->CMS, ctx->nchem, ctx->queue, ctx->cnt,
-                                                              keep);
+        sort_queue(ctx);
+        k_chem_queue<NS, MT><<<ctx->ncu * 4, kBlock, 0, ctx->stream>>>(g, ctx->mech, dt, ctx->CS, ctx->prim, ctx->T,
+                                                                       ctx->E, ctx->C, ctx->CMS, ctx->nchem,
+                                                                       ctx->queue2, ctx->cnt, keep);
     } else {
         k_chem_all<NS, MT><<<blocks_for(ctx->n_int), kBlock, 0, ctx->stream>>>(g, ctx->mech, dt, ctx->CS, ctx->prim,
                                                                            ctx->T, ctx->E, ctx->C, ctx->CMS,
@@ -1373,8 +1491,9 @@ static void launch_chem_sur(eu3d_ctx *ctx, double dt, int flags, int *rc)
             g, ctx->mech, ctx->dsur, ctx->sur_list + off, count, dt, v.Y, v.norm, ctx->sur_ok, ctx->CS, ctx->prim,
             ctx->T, ctx->E, ctx->C, ctx->CMS, keep, ctx->queue, ctx->cnt);
     }
+    sort_queue(ctx);
     k_chem_queue<NS, MT, true><<<ctx->ncu * 4, kBlock, 0, ctx->stream>>>(
-        g, ctx->mech, dt, ctx->CS, ctx->prim, ctx->T, ctx->E, ctx->C, ctx->CMS, ctx->nchem, ctx->queue, ctx->cnt, keep);
+        g, ctx->mech, dt, ctx->CS, ctx->prim, ctx->T, ctx->E, ctx->C, ctx->CMS, ctx->nchem, ctx->queue2, ctx->cnt, keep);
     k_chem_health<<<blocks_for(ctx->n_int), kBlock, 0, ctx->stream>>>(g, ctx->prim, ctx->T, ctx->cnt);
 }
 
@@ -1484,7 +1603,7 @@ int eu3d_destroy(eu3d_ctx *ctx)
         (void)hipEventDestroy(e);
     void *ptrs[] = {ctx->mech,  ctx->x,      ctx->y,     ctx->z,       ctx->prim,    ctx->T,
                     ctx->E,     ctx->C,      ctx->CS,    ctx->CS0,     ctx->CMS,     ctx->scratch,
-                    ctx->nchem, ctx->queue,  ctx->cnt,   ctx->dsur,    ctx->sur_list, ctx->inflow,
+                    ctx->nchem, ctx->queue,  ctx->cnt,   ctx->dsur,    ctx->sur_list, ctx->inflow, ctx->queue2,
                     ctx->sur_ok};
     for (void *p : ptrs)
         if (p)
@@ -1587,6 +1706,7 @@ int eu3d_create(const eu3d_mesh *mesh, const eu3d_mechanism *mc, eu3d_ctx **out)
     CK(hipMalloc(&ctx->scratch, sizeof(double) * N));
     CK(hipMalloc(&ctx->nchem, sizeof(int) * N));
     CK(hipMalloc(&ctx->queue, sizeof(int) * (size_t)ctx->n_int));
+    CK(hipMalloc(&ctx->queue2, sizeof(int) * (size_t)ctx->n_int));
     CK(hipMalloc(&ctx->cnt, sizeof(ChemCounters)));
     CK(hipMemset(ctx->cnt, 0, sizeof(ChemCounters)));
     CK(hipMemset(ctx->CMS, 0, sizeof(double) * N * ns));
@@ -2109,8 +2229,12 @@ int eu3d_health(eu3d_ctx *ctx, eu3d_health_info *out)
     out->max_nchem = c.max_nchem;
     out->stiff_cells = (long long)(c.queue_count - c.fb_count);
     out->total_substeps = (long long)c.substeps;
-    if (c.invalid || c.nonfinite) {
-        ctx->err = "invalid Nchem or non-finite state in the last chemistry call";
+    if (c.sticky_bad)
+        CK(hipMemsetAsync(&ctx->cnt->sticky_bad, 0, sizeof c.sticky_bad, ctx->stream));
+    if (c.invalid || c.nonfinite || c.sticky_bad) {
+        ctx->err = c.invalid || c.nonfinite
+                       ? "invalid Nchem or non-finite state in the last chemistry call"
+                       : "invalid Nchem or non-finite state in a chemistry call since the last health check";
         return EU3D_ERR_HEALTH;
     }
     return EU3D_OK;

@@ -829,9 +829,12 @@ void Euler::Computing(int max_steps)
         time += TwoDim.dt;
         iteration += 1;
         n++;
-        // the reference's per-step resident health check and abort
-        // (src/ReactionCuda.cu:2460-2465): invalid Nchem or a non-finite cell
-        if (!(chem_flags & EU3D_CHEM_SKIP)) {
+        // the reference's resident health check and abort (src/ReactionCuda.cu:
+        // 2460-2465): invalid Nchem or a non-finite cell.  The reference checks
+        // every step; here every 16 steps and at the end of the call, so that
+        // the step loop does not wait for the device each step.
+        if (!(chem_flags & EU3D_CHEM_SKIP) &&
+            ((iteration % 16) == 0 || n == max_steps || Final_Time - time <= time_tol)) {
             eu3d_health_info h;
             check(eu3d_health(TwoDim.ctx, &h), TwoDim.ctx, "eu3d_health");
         }

@@ -150,7 +150,9 @@ int eu3d_run(eu3d_ctx *ctx, int scheme, double cfl, double dt_fixed, double fina
              int chem_flags, double *time, int *steps_done);
 
 /* Health counters of the last chemistry call; returns EU3D_ERR_HEALTH if
- * invalid Nchem or non-finite cells were found. */
+ * invalid Nchem or non-finite cells (density or temperature, as
+ * resident_health_kernel, src/ReactionCuda.cu:2265-2273) were found in the
+ * last call or in any call since the previous eu3d_health. */
 int eu3d_health(eu3d_ctx *ctx, eu3d_health_info *out);
 
 /* Options.  EU3D_OPT_GENERIC_MECHANISM (read/write): 1 runs the chemistry and

@@ -287,6 +287,24 @@ def test_restart_continues_bitwise(tmp_path, case):
     assert open(pa, "rb").read() == open(pc, "rb").read()
 
 
+def test_health_flags_nonfinite_and_invalid_cells(tmp_path):
+    """resident_health_kernel's checks: a cell given a NaN temperature from
+    the host is counted as non-finite (its rates are NaN, so the prediction's
+    comparisons fail and Nchem stays 1, as in TrapezoidPrediction); the
+    driver's run stops with EU3D_ERR_HEALTH instead of continuing."""
+    ctrl = cases.rsbi(str(tmp_path), steps=3)
+    s = eu3d.Solver(ctrl)
+    s.interior(s.field("T"))[2, 3, 1] = np.nan
+    s.upload()
+    L = eu3d.load()
+    assert L.eu3d_chemistry(s.ctx, 3e-8, eu3d.CHEM_COMPACT) == 0
+    rc, h = s.health()
+    assert rc == eu3d.EU3D_ERR_HEALTH
+    assert h.nonfinite >= 1 and h.invalid_nchem == 0
+    with pytest.raises(eu3d.EU3DError):
+        s.run(3)
+
+
 def test_generated_mechanism_equals_runtime_tables(tmp_path):
     """The compile-time mechanism kernels and the run-time-table kernels give
     the same bits (stiff RSBI, 6 steps)."""
