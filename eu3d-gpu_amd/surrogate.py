@@ -372,6 +372,37 @@ def znd_dataset(steps, min_nchem=2, ctrl=None, log=None):
     return np.concatenate(xs), np.concatenate(ts)
 
 
+def odw_dataset(develop=3000, steps=10, keep=4_000_000, seed=5, min_nchem=2, log=None):
+    """(features, targets) of the stiff cells of the developed oblique
+    detonation (input/ctrl-3dODW.txt): `develop` FP64 steps, then `steps`
+    harvested steps, about `keep` rows kept in a seeded random subsample."""
+    s = _pkg.Solver(os.path.join(HERE, "input", "ctrl-3dODW.txt"), base_dir=HERE)
+    flags = _pkg.CHEM_COMPACT | _pkg.CHEM_KEEP_CMS
+    done = 0
+    while done < develop:
+        k = min(500, develop - done)
+        s.run(k, flags)
+        done += k
+        if log:
+            log(f"ODW developed {done} steps")
+    ns, n = s.ns, s.ni * s.nj * s.nk
+    rng = np.random.default_rng(seed)
+    xs, ts = [], []
+
+    def chem(ctx, dt):
+        x, t, rows = harvest_call(ctx, ns, dt, flags, min_nchem, n)
+        pick = rng.random(len(x)) < keep / (steps * max(len(x), 1))
+        xs.append(x[pick])
+        ts.append(t[pick])
+
+    march(s, steps, chem, t0=s.info()["time"])
+    s.close()
+    x, t = np.concatenate(xs), np.concatenate(ts)
+    if log:
+        log(f"ODW harvest: {steps} steps, {len(x)} rows kept")
+    return x, t
+
+
 def front_index(s, p_ref=2.0 * 101325.0):
     """Leading-shock cell of a 1-D run: the largest i with p > p_ref on the
     centre line (interior indices)."""
@@ -487,7 +518,7 @@ RATE_DT = 1e-8  # targets are learned as changes per 10 ns (scaled by RATE_DT / 
 
 
 def train_recipe(out, n_states=1 << 18, calls=8, znd_steps=3000, epochs=600, batch=16384, seed=20240601,
-                 log=print):
+                 log=print, odw=False, odw_epochs=300):
     """The committed model's recipe: 0-D ignition data (TRAIN_STATES,
     TRAIN_DT) plus the stiff cells of a ZND initiation run, normalization
     from the set, Adam with a cosine learning rate, training box as the hull."""
@@ -507,6 +538,10 @@ def train_recipe(out, n_states=1 << 18, calls=8, znd_steps=3000, epochs=600, bat
     x1, t1_ = znd_dataset(znd_steps, log=log)
     log(f"ZND harvest in {time.time() - t1:.1f} s")
     x, t = np.concatenate([x0, x1]), np.concatenate([t0_, t1_])
+    if odw:  # plus the developed oblique detonation's stiff cells
+        x2, t2 = odw_dataset(log=log)
+        x, t = np.concatenate([x, x2]), np.concatenate([t, t2])
+        epochs = odw_epochs
     keep = np.isfinite(x).all(1) & np.isfinite(t).all(1)
     if not keep.all():
         log(f"dropped {np.count_nonzero(~keep)} non-finite rows")
@@ -535,13 +570,15 @@ def main():
     tr.add_argument("--calls", type=int, default=8)
     tr.add_argument("--znd-steps", type=int, default=3000)
     tr.add_argument("--epochs", type=int, default=600)
+    tr.add_argument("--odw", action="store_true", help="add 4 M stiff cells of the developed oblique detonation "
+                                                         "(300 epochs)")
     ev = sub.add_parser("eval")
     ev.add_argument("--model", default=DEFAULT_MODEL)
     ev.add_argument("--znd-steps", type=int, default=2000)
     ev.add_argument("--tol-T", type=float, default=ACCEPT["tol_T"])
     a = ap.parse_args()
     if a.cmd == "train":
-        train_recipe(a.out, n_states=a.states, calls=a.calls, znd_steps=a.znd_steps, epochs=a.epochs)
+        train_recipe(a.out, n_states=a.states, calls=a.calls, znd_steps=a.znd_steps, epochs=a.epochs, odw=a.odw)
     else:
         import json
         names = species_names(os.path.join(HERE, "input", "H2_JA919.txt"))
