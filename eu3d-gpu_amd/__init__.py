@@ -27,6 +27,7 @@ CHEM_COMPACT = 0x1
 CHEM_SKIP = 0x2
 CHEM_KEEP_CMS = 0x4
 CHEM_SURROGATE = 0x8
+CHEM_FIXED = 0x10
 BOUNDARY_RSBI = 0
 BOUNDARY_ODW = 1
 OPT_GENERIC_MECHANISM = 1
@@ -99,6 +100,7 @@ SIGNATURES = {
     "eu3d_run": (_i, [_vp, _i, _d, _d, _d, _i, _i, ctypes.POINTER(_d), ctypes.POINTER(_i)]),
     "eu3d_health": (_i, [_vp, ctypes.POINTER(Health)]),
     "eu3d_synchronize": (_i, [_vp]),
+    "eu3d_max_nchem_log": (_i, [_vp, _vp, _i, ctypes.POINTER(_i)]),
     "eu3d_set_option": (_i, [_vp, _i, _i]),
     "eu3d_set_boundary": (_i, [_vp, _i, _vp]),
     "eu3d_get_boundary": (_i, [_vp, ctypes.POINTER(_i)]),
@@ -123,6 +125,8 @@ SIGNATURES = {
     "eu3d_solver_dump": (_i, [_vp, ctypes.c_char_p, _i]),
     "eu3d_solver_error": (ctypes.c_char_p, [_vp]),
     "eu3d_solver_restart": (_i, [_vp, ctypes.c_char_p, _d]),
+    "eu3d_solver_set_output": (_i, [_vp, ctypes.c_char_p]),
+    "eu3d_solver_output": (_i, [_vp, _i]),
     "eu3d_mlp_create": (_i, [_i, _i, _i, _i, ctypes.c_ulonglong, ctypes.POINTER(_vp)]),
     "eu3d_mlp_destroy": (_i, [_vp]),
     "eu3d_mlp_last_error": (ctypes.c_char_p, [_vp]),
@@ -229,6 +233,14 @@ class Solver:
 
     def dump(self, path, num=1):
         self._check(self.L.eu3d_solver_dump(self.h, path.encode(), num), "dump")
+
+    def set_output(self, directory):
+        """Enable the reference's output tree under directory (Tecplot series, load logs)."""
+        self._check(self.L.eu3d_solver_set_output(self.h, directory.encode()), "set_output")
+
+    def output(self, num):
+        """Write output number num now (Euler::Output)."""
+        self._check(self.L.eu3d_solver_output(self.h, num), "output")
 
     def restart(self, path, time=-1.0):
         """Restore the state of an EU3D_STATE_V1_LE dump (eu3d_solver_restart)."""

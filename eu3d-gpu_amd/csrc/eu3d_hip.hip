@@ -26,6 +26,7 @@ using namespace eu3d;
 namespace {
 
 constexpr int kBlock = 256;
+constexpr int kNchemLog = 1 << 16;
 
 struct ChemCounters {
     unsigned int queue_count;
@@ -84,6 +85,10 @@ struct eu3d_ctx {
     int *sur_list;
     unsigned char *sur_ok;
     int sur_ok_cap;
+    // max Nchem of every chemistry call (Reaction::PushNchemMax), a ring of
+    // kNchemLog entries read by eu3d_max_nchem_log
+    int *nchem_log;
+    long long log_written, log_read;
     // ghost-fill set (EU3D_BOUNDARY_*) and the inflow state of the ODW set
     int boundary;
     double *inflow; // nprim + 2: P D U V W H Gamma Y.. T E
@@ -850,6 +855,9 @@ __global__ void __launch_bounds__(kBlock)
     }
 }
 
+// Reaction::PushNchemMax (src/Reaction.cpp:953-957): the call's max Nchem
+__global__ void k_log_max_nchem(const ChemCounters *cnt, int *slot) { *slot = cnt->max_nchem; }
+
 // Remaining sub-steps 1..Nchem-1 of one cell.
 template <int NS, class MT>
 __device__ __forceinline__ void chem_rest(const Grid &g, const MT &m, long long c, int n, double dtm,
@@ -988,6 +996,33 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(EU3
                 step++;
             }
         }
+    }
+}
+
+// Euler::Trapezoid(type 0): one trapezoid sub-step of the full dt in every
+// interior cell with the rates at its state, no prediction
+// (src/Reaction.cpp:485-559), then the state rebuilt and recovered.  The
+// reference's Flowfield::Explicit() for this type writes the energy into the
+// w-momentum slot (src/Flowfield.cpp:1514-1516); the rebuild here is the one
+// of Explicit(step, i, j, k), with both in their own slots.
+template <int NS, class MT>
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(EU3D_CHEM_WAVES_PER_EU)))
+    k_chem_fixed(Grid g, const DevMech *__restrict__ mp, double dtm, double *__restrict__ CS,
+                 double *__restrict__ prim, double *__restrict__ T, double *__restrict__ E, double *__restrict__ C,
+                 double *__restrict__ CMS, int *__restrict__ nchem, ChemCounters *cnt, int keep_cms)
+{
+    long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    int i = 0, j = 0, k = 0;
+    const bool active = interior_cell(g, t, i, j, k);
+    if (active) {
+        const long long c = gidx(g, i, j, k);
+        chem_step_cell<NS, false>(g, mech_ref<MT>(mp), c, dtm, 1, CS, prim, T, E, C, CMS, keep_cms);
+        nchem[c] = 1;
+    }
+    const unsigned long long m = __ballot(active);
+    if ((threadIdx.x & 63) == 0 && m) {
+        atomicAdd(&cnt->substeps, (unsigned long long)__popcll(m));
+        atomicMax(&cnt->max_nchem, 1);
     }
 }
 
@@ -1437,6 +1472,12 @@ static void launch_chem(eu3d_ctx *ctx, double dt, int flags)
     int compact = (flags & EU3D_CHEM_COMPACT) ? 1 : 0;
     int keep = (flags & EU3D_CHEM_KEEP_CMS) ? 1 : 0;
     k_chem_reset<<<1, 1, 0, ctx->stream>>>(ctx->cnt);
+    if (flags & EU3D_CHEM_FIXED) {
+        k_chem_fixed<NS, MT><<<blocks_for(ctx->n_int), kBlock, 0, ctx->stream>>>(
+            g, ctx->mech, dt, ctx->CS, ctx->prim, ctx->T, ctx->E, ctx->C, ctx->CMS, ctx->nchem, ctx->cnt, keep);
+        k_chem_health<<<blocks_for(ctx->n_int), kBlock, 0, ctx->stream>>>(g, ctx->prim, ctx->T, ctx->cnt);
+        return;
+    }
     k_chem_first<NS, MT, false><<<blocks_for(ctx->n_int), kBlock, 0, ctx->stream>>>(
         g, ctx->mech, dt, ctx->CS, ctx->prim, ctx->T, ctx->E, ctx->C, ctx->CMS, ctx->nchem, ctx->queue, ctx->cnt,
         compact, keep, 0, nullptr);
@@ -1603,7 +1644,7 @@ int eu3d_destroy(eu3d_ctx *ctx)
         (void)hipEventDestroy(e);
     void *ptrs[] = {ctx->mech,  ctx->x,      ctx->y,     ctx->z,       ctx->prim,    ctx->T,
                     ctx->E,     ctx->C,      ctx->CS,    ctx->CS0,     ctx->CMS,     ctx->scratch,
-                    ctx->nchem, ctx->queue,  ctx->cnt,   ctx->dsur,    ctx->sur_list, ctx->inflow, ctx->queue2,
+                    ctx->nchem, ctx->queue,  ctx->cnt,   ctx->dsur,    ctx->sur_list, ctx->inflow, ctx->queue2, ctx->nchem_log,
                     ctx->sur_ok};
     for (void *p : ptrs)
         if (p)
@@ -1707,6 +1748,7 @@ int eu3d_create(const eu3d_mesh *mesh, const eu3d_mechanism *mc, eu3d_ctx **out)
     CK(hipMalloc(&ctx->nchem, sizeof(int) * N));
     CK(hipMalloc(&ctx->queue, sizeof(int) * (size_t)ctx->n_int));
     CK(hipMalloc(&ctx->queue2, sizeof(int) * (size_t)ctx->n_int));
+    CK(hipMalloc(&ctx->nchem_log, sizeof(int) * kNchemLog));
     CK(hipMalloc(&ctx->cnt, sizeof(ChemCounters)));
     CK(hipMemset(ctx->cnt, 0, sizeof(ChemCounters)));
     CK(hipMemset(ctx->CMS, 0, sizeof(double) * N * ns));
@@ -1971,6 +2013,8 @@ int eu3d_advance_rk3(eu3d_ctx *ctx, double dt)
     return EU3D_OK;
 }
 
+static int chemistry_dispatch(eu3d_ctx *ctx, double dt, int flags);
+
 int eu3d_chemistry(eu3d_ctx *ctx, double dt, int flags)
 {
     if (!ctx)
@@ -1981,6 +2025,37 @@ int eu3d_chemistry(eu3d_ctx *ctx, double dt, int flags)
         FAIL(EU3D_ERR_ARG, "dt must be > 0");
     if (flags & EU3D_CHEM_SKIP)
         return EU3D_OK;
+    int rc = chemistry_dispatch(ctx, dt, flags);
+    if (rc)
+        return rc;
+    k_log_max_nchem<<<1, 1, 0, ctx->stream>>>(ctx->cnt, ctx->nchem_log + ctx->log_written % kNchemLog);
+    ctx->log_written++;
+    CK(hipGetLastError());
+    return EU3D_OK;
+}
+
+int eu3d_max_nchem_log(eu3d_ctx *ctx, int *out, int capacity, int *count)
+{
+    if (!ctx || !count || capacity < 0 || (capacity > 0 && !out))
+        return EU3D_ERR_ARG;
+    long long from = std::max(ctx->log_read, ctx->log_written - kNchemLog);
+    long long n = std::min<long long>(ctx->log_written - from, capacity);
+    std::vector<int> ring(kNchemLog);
+    if (n > 0) {
+        CK(hipMemcpyAsync(ring.data(), ctx->nchem_log, sizeof(int) * kNchemLog, hipMemcpyDeviceToHost, ctx->stream));
+        CK(hipStreamSynchronize(ctx->stream));
+        for (long long q = 0; q < n; q++)
+            out[q] = ring[(from + q) % kNchemLog];
+    }
+    ctx->log_read = from + n;
+    *count = (int)n;
+    return EU3D_OK;
+}
+
+static int chemistry_dispatch(eu3d_ctx *ctx, double dt, int flags)
+{
+    if ((flags & EU3D_CHEM_SURROGATE) && (flags & EU3D_CHEM_FIXED))
+        FAIL(EU3D_ERR_ARG, "EU3D_CHEM_SURROGATE applies to the adaptive trapezoid only");
     if (flags & EU3D_CHEM_SURROGATE) {
         if (!ctx->mlp)
             FAIL(EU3D_ERR_STATE, "EU3D_CHEM_SURROGATE without an attached network (eu3d_set_surrogate)");

@@ -6,6 +6,7 @@
 #include "eu3d_host.hpp"
 
 #include <cerrno>
+#include <sys/stat.h>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -767,8 +768,14 @@ Euler::Euler(const std::string &ctrlfile, const std::string &base_dir, bool devi
     if (cfl == 0)
         dt = Ctrl.getdoubleParameter("dt");
     count = Ctrl.getIntParameter("count");
-    if (Ctrl.getStringParameter("reaction") != "Trapezoid-Adaptive")
-        throw Error(EU3D_ERR_ARG, "reaction scheme must be Trapezoid-Adaptive");
+    const std::string rs = Ctrl.getStringParameter("reaction");
+    react_name = rs;
+    if (rs == "Trapezoid-Adaptive")
+        Reaction_type = 1;
+    else if (rs == "Trapezoid")
+        Reaction_type = 0;
+    else
+        throw Error(EU3D_ERR_ARG, "reaction scheme must be Trapezoid or Trapezoid-Adaptive");
     if (Ctrl.getStringParameter("difference") != "MUSCL_2")
         throw Error(EU3D_ERR_ARG, "difference scheme must be MUSCL_2");
     std::string ta = Ctrl.getStringParameter("time-adv");
@@ -825,7 +832,14 @@ void Euler::Computing(int max_steps)
             TwoDim.CFLcondition(cfl, Final_Time);
         if (TwoDim.dt > Final_Time - time)
             TwoDim.dt = Final_Time - time;
-        check(eu3d_step(TwoDim.ctx, TimeAdv_Sch, TwoDim.dt, chem_flags), TwoDim.ctx, "eu3d_step");
+        // real-time output at uniform times (src/Euler.cpp:331-339)
+        if (!out_dir.empty() && time <= out_num * Final_Time / count &&
+            time + TwoDim.dt > out_num * Final_Time / count) {
+            Output(out_num);
+            out_num = out_num + 1;
+        }
+        const int flags = chem_flags | (Reaction_type == 0 ? EU3D_CHEM_FIXED : 0);
+        check(eu3d_step(TwoDim.ctx, TimeAdv_Sch, TwoDim.dt, flags), TwoDim.ctx, "eu3d_step");
         time += TwoDim.dt;
         iteration += 1;
         n++;
@@ -833,8 +847,13 @@ void Euler::Computing(int max_steps)
         // 2460-2465): invalid Nchem or a non-finite cell.  The reference checks
         // every step; here every 16 steps and at the end of the call, so that
         // the step loop does not wait for the device each step.
+        LoadIterations.push_back(iteration);
         if (!(chem_flags & EU3D_CHEM_SKIP) &&
             ((iteration % 16) == 0 || n == max_steps || Final_Time - time <= time_tol)) {
+            int got = 0;
+            std::vector<int> mx(1 << 16);
+            check(eu3d_max_nchem_log(TwoDim.ctx, mx.data(), (int)mx.size(), &got), TwoDim.ctx, "eu3d_max_nchem_log");
+            NchemMax.insert(NchemMax.end(), mx.begin(), mx.begin() + got);
             eu3d_health_info h;
             check(eu3d_health(TwoDim.ctx, &h), TwoDim.ctx, "eu3d_health");
         }
@@ -889,6 +908,82 @@ void Euler::WriteStateDump(const std::string &p, int num)
             }
     if (std::fclose(f) != 0)
         throw Error(EU3D_ERR_ARG, "dump write failed " + p);
+}
+
+static void make_dirs(const std::string &path)
+{
+    std::string cur;
+    std::istringstream parts(path);
+    if (!path.empty() && path[0] == '/')
+        cur = "/";
+    for (std::string p; std::getline(parts, p, '/');) {
+        if (p.empty())
+            continue;
+        cur += p + "/";
+        if (mkdir(cur.c_str(), 0755) != 0 && errno != EEXIST)
+            throw Error(EU3D_ERR_ARG, "cannot create directory " + cur);
+    }
+}
+
+// Euler::Output (src/Euler.cpp:1536-1795), one process: the Tecplot block
+// file "Series<reaction><num>_0.dat" with the ghost nodes and the variables
+// X Y Z T D P U V Ma Y1..Y7 in the reference's default stream formatting;
+// the EU3D_DUMP_STATE dump when num == count; the load logs so far.
+void Euler::Output(int num)
+{
+    if (out_dir.empty())
+        throw Error(EU3D_ERR_STATE, "no output directory set");
+    if (TwoDim.ctx)
+        SyncOutput();
+    const std::string res = out_dir + "/output_1/results";
+    make_dirs(res);
+    const char *dump_requested = std::getenv("EU3D_DUMP_STATE");
+    if (dump_requested && std::strcmp(dump_requested, "1") == 0 && num == count)
+        WriteStateDump(res + "/state_check_" + std::to_string(num) + "_rank0.bin", num);
+    const std::string name = res + "/Series" + react_name + std::to_string(num) + "_0.dat";
+    std::ofstream out(name);
+    if (!out)
+        throw Error(EU3D_ERR_ARG, "cannot open " + name);
+    const Flowfield &F = TwoDim;
+    const int NI = F.NI, NJ = F.NJ, NK = F.NK, NS = React.NS;
+    out << "Variables = X,Y,Z,T,D,P,U,V,Ma,Y1,Y2,Y3,Y4,Y5,Y6,Y7\n";
+    out << "ZONE I=" << NI << '\t' << "J=" << NJ << '\t' << "K=" << NK << '\n';
+    out << "datapacking=block\n";
+    auto block = [&](auto &&value) {
+        for (int k = 0; k < NK; k++)
+            for (int j = 0; j < NJ; j++)
+                for (int i = 0; i < NI; i++)
+                    out << value(i, j, k, F.idx(i, j, k)) << '\n';
+    };
+    block([&](int i, int, int, size_t) { return Mymesh.xnode[i]; });
+    block([&](int, int j, int, size_t) { return Mymesh.ynode[j]; });
+    block([&](int, int, int k, size_t) { return Mymesh.znode[k]; });
+    for (const std::vector<double> *v : {&F.T, &F.D, &F.P, &F.U, &F.V, &F.Ma})
+        block([&](int, int, int, size_t id) { return (*v)[id]; });
+    for (int q = 0; q < 7 && q < NS; q++)
+        block([&](int, int, int, size_t id) { return F.Yi[id * NS + q]; });
+    if (!out)
+        throw Error(EU3D_ERR_ARG, "write failed " + name);
+    out.close();
+    WriteLoadLogs();
+}
+
+// Load_Loop.dat (Reaction::GetNchemMax, src/Reaction.cpp:967-988): step and
+// max Nchem of that step; LoadDegree.dat (src/Euler.cpp:318, 400): iteration,
+// LoadDegree, LoadDegreeBalance -- (max - mean) / max of the per-rank Nchem
+// totals, 0 with the single rank here.
+void Euler::WriteLoadLogs()
+{
+    const std::string dir = out_dir + "/output_1/load";
+    make_dirs(dir);
+    std::ofstream ll(dir + "/Load_Loop.dat");
+    for (size_t i = 1; i <= NchemMax.size(); i++)
+        ll << i << '\t' << NchemMax[i - 1] << '\n';
+    std::ofstream ld(dir + "/LoadDegree.dat");
+    for (int it : LoadIterations)
+        ld << it << '\t' << 0.0 << '\t' << 0.0 << '\n';
+    if (!ll || !ld)
+        throw Error(EU3D_ERR_ARG, "cannot write the load logs under " + dir);
 }
 
 // Restart from an EU3D_STATE_V1_LE dump of the same grid and mechanism (no

@@ -136,11 +136,24 @@ class Euler {
     void SyncOutput();
     void WriteStateDump(const std::string &path, int num);
     void ReadStateDump(const std::string &path, double restart_time);
+    // Euler::Output (src/Euler.cpp:1536-1795): Tecplot Series file of output
+    // number num under <out_dir>/output_1/results, the state dump when
+    // EU3D_DUMP_STATE=1 and num == count, and the load logs so far.
+    void Output(int num);
+    void SetOutput(const std::string &dir) { out_dir = dir; }
+    void WriteLoadLogs();
+
+    std::string out_dir;    // empty: no scheduled output
+    std::string react_name; // ctrl "reaction" keyword (file names)
+    int out_num = 0;        // next scheduled output (src/Euler.cpp:313)
+    std::vector<int> NchemMax;        // Load_Loop.dat (Reaction::GetNchemMax)
+    std::vector<int> LoadIterations;  // LoadDegree.dat rows
     void SetChemFlags(int f) { chem_flags = f; }
 
     std::string base;
     double Final_Time = 0, cfl = 0, dt = 0;
     int count = 0, TimeAdv_Sch = 0, iteration = 0;
+    int Reaction_type = 1; // ctrl "reaction": Trapezoid = 0, Trapezoid-Adaptive = 1 (src/Euler.cpp:136-160)
     int chem_flags = EU3D_CHEM_COMPACT | EU3D_CHEM_KEEP_CMS;
     double time = 0;
     Mesh Mymesh;

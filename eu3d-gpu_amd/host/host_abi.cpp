@@ -152,6 +152,20 @@ int eu3d_solver_dump(eu3d_solver *s, const char *path, int num)
     GUARD(s->e->WriteStateDump(path, num));
 }
 
+int eu3d_solver_set_output(eu3d_solver *s, const char *dir)
+{
+    if (!s || !dir)
+        return EU3D_ERR_ARG;
+    GUARD(s->e->SetOutput(dir));
+}
+
+int eu3d_solver_output(eu3d_solver *s, int num)
+{
+    if (!s)
+        return EU3D_ERR_ARG;
+    GUARD(s->e->Output(num < 0 ? s->e->count : num));
+}
+
 int eu3d_solver_restart(eu3d_solver *s, const char *path, double time)
 {
     if (!s || !path)

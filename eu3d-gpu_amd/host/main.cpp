@@ -12,7 +12,7 @@
 
 int main(int argc, char **argv)
 {
-    std::string ctrl = "./input/ctrl.txt", base = ".", dump;
+    std::string ctrl = "./input/ctrl.txt", base = ".", dump, outdir;
     int steps = 1 << 30, flags = EU3D_CHEM_COMPACT | EU3D_CHEM_KEEP_CMS;
     for (int a = 1; a < argc; a++) {
         std::string s = argv[a];
@@ -22,6 +22,8 @@ int main(int argc, char **argv)
             base = argv[++a];
         else if (s == "--dump" && a + 1 < argc)
             dump = argv[++a];
+        else if (s == "--output" && a + 1 < argc)
+            outdir = argv[++a];
         else if (s == "--no-compact")
             flags &= ~EU3D_CHEM_COMPACT;
         else
@@ -35,6 +37,11 @@ int main(int argc, char **argv)
     int rc = eu3d_solver_open(ctrl.c_str(), base.c_str(), &s, err, sizeof err);
     if (rc) {
         std::fprintf(stderr, "eu3d: %s\n", err);
+        return 2;
+    }
+    if (!outdir.empty() && (rc = eu3d_solver_set_output(s, outdir.c_str()))) {
+        std::fprintf(stderr, "eu3d: %s\n", eu3d_solver_error(s));
+        eu3d_solver_close(s);
         return 2;
     }
     auto t0 = std::chrono::steady_clock::now();
@@ -60,6 +67,12 @@ int main(int argc, char **argv)
         rc = eu3d_solver_dump(s, dump.c_str(), 1);
         if (rc)
             std::fprintf(stderr, "eu3d: %s\n", eu3d_solver_error(s));
+    }
+    if (!outdir.empty()) { // the reference's final Output(count) (src/Euler.cpp:101)
+        int rc2 = eu3d_solver_output(s, -1);
+        if (rc2)
+            std::fprintf(stderr, "eu3d: %s\n", eu3d_solver_error(s));
+        rc = rc ? rc : rc2;
     }
     eu3d_solver_close(s);
     return hr ? 4 : (rc ? 3 : 0);

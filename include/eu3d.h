@@ -38,6 +38,8 @@ extern "C" {
 #define EU3D_CHEM_COMPACT 0x1   /* stiff cells (Nchem > 1) are binned into a dense work queue */
 #define EU3D_CHEM_SKIP 0x2      /* advection only (no chemistry) */
 #define EU3D_CHEM_KEEP_CMS 0x4  /* store the last sub-step mass production rates (dump field CMS) */
+#define EU3D_CHEM_FIXED 0x10    /* Euler::Trapezoid(type 0) (ctrl "reaction Trapezoid"): one sub-step of the
+                                   full dt in every cell, no Nchem prediction (src/Reaction.cpp:485-559) */
 
 typedef struct eu3d_ctx eu3d_ctx;
 
@@ -165,6 +167,11 @@ int eu3d_health(eu3d_ctx *ctx, eu3d_health_info *out);
 #define EU3D_OPT_GENERATED_AVAILABLE 2
 int eu3d_set_option(eu3d_ctx *ctx, int option, int value);
 int eu3d_get_option(eu3d_ctx *ctx, int option, int *value);
+
+/* Reaction::PushNchemMax (src/Reaction.cpp:953-957): the max Nchem of each
+ * chemistry call since the previous read, oldest first (the device keeps the
+ * last 65536); *count entries written to out (at most capacity). */
+int eu3d_max_nchem_log(eu3d_ctx *ctx, int *out, int capacity, int *count);
 
 int eu3d_synchronize(eu3d_ctx *ctx);
 int eu3d_timing_get(eu3d_ctx *ctx, eu3d_timing *out);

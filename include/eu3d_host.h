@@ -44,6 +44,14 @@ double *eu3d_solver_field(eu3d_solver *s, const char *name);
 int *eu3d_solver_nchem(eu3d_solver *s);
 /* EU3D_STATE_V1_LE dump (src/Euler.cpp:1539-1598) */
 int eu3d_solver_dump(eu3d_solver *s, const char *path, int num);
+/* Euler::Output (src/Euler.cpp:1536-1795): set_output enables the
+ * reference's output tree under dir (output_1/results/Series<reaction><num>_0.dat
+ * Tecplot files at Final_Time/count intervals during eu3d_solver_run, the
+ * EU3D_DUMP_STATE dump, output_1/load/Load_Loop.dat and LoadDegree.dat);
+ * eu3d_solver_output writes output number num now; num < 0 is the ctrl's
+ * count, the reference's final Output(count) (src/Euler.cpp:101). */
+int eu3d_solver_set_output(eu3d_solver *s, const char *dir);
+int eu3d_solver_output(eu3d_solver *s, int num);
 /* Restart from such a dump of the same grid and mechanism (no reference
  * counterpart: the reference writes the dump only).  Restores the interior
  * state, Nchem, CMS and the iteration counter, and uploads; the time is

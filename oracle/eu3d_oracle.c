@@ -884,8 +884,13 @@ or_state *or_init(const char *ctrl_path, const char *base_dir, char *err, int er
     s->count = kf_int(ctrl, "count", &ok);
     {
         const char *r = kf_get(ctrl, "reaction"), *d = kf_get(ctrl, "difference"), *t = kf_get(ctrl, "time-adv");
-        if (!r || strcmp(r, "Trapezoid-Adaptive") != 0) {
-            snprintf(err, errlen, "reaction scheme must be Trapezoid-Adaptive");
+        /* Euler::FileRead (src/Euler.cpp:136-160): Trapezoid = type 0, Trapezoid-Adaptive = type 1 */
+        if (r && strcmp(r, "Trapezoid-Adaptive") == 0)
+            s->reaction_type = 1;
+        else if (r && strcmp(r, "Trapezoid") == 0)
+            s->reaction_type = 0;
+        else {
+            snprintf(err, errlen, "reaction scheme must be Trapezoid or Trapezoid-Adaptive");
             goto fail;
         }
         if (!d || strcmp(d, "MUSCL_2") != 0) {
@@ -901,7 +906,6 @@ or_state *or_init(const char *ctrl_path, const char *base_dir, char *err, int er
             goto fail;
         }
         s->diff = 1;
-        s->reaction_type = 1;
     }
     const char *gname = kf_get(ctrl, "gridname"), *iname = kf_get(ctrl, "initialization"),
                *mname = kf_get(ctrl, "reaction-model"), *tname = kf_get(ctrl, "Thermofile");
@@ -1593,6 +1597,29 @@ void or_chemistry(or_state *s)
             }
 }
 
+/* Euler::Trapezoid(type 0): Reaction::Trapezoid without the prediction
+ * (src/Reaction.cpp:485-559), one sub-step of the full dt in every cell, then
+ * Flowfield::Explicit() + Update_after_CS() (src/Flowfield.cpp:1502-1518).
+ * The reference's Explicit() writes the energy into the w-momentum slot
+ * CS(NS+2) and leaves CS(NS+3) unchanged; here the conserved vector is
+ * rebuilt as in Explicit(step, i, j, k) (src/Flowfield.cpp:1522-1536), with
+ * w-momentum and energy in their own slots.  Nchem is 1 everywhere. */
+void or_chemistry_fixed(or_state *s)
+{
+    int ns = s->ns, bc = s->bc;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static)
+#endif
+    for (int i = bc; i < s->ni + bc; i++)
+        for (int j = bc; j < s->nj + bc; j++)
+            for (int k = bc; k < s->nk + bc; k++) {
+                size_t id = IDX(s, i, j, k);
+                s->Nchem[id] = 1;
+                or_trapezoid_cell(s, s->T[id], s->Mc + id * ns, s->Di + id * ns, s->dt, s->CMS + id * ns);
+                explicit_cell(s, id);
+            }
+}
+
 /* ====================================================================== */
 /* SSP-RK3 advection (new; TimeAdv::TVD_RK3 is a no-op in the reference,  */
 /* src/TimeAdv.cpp:210-221).  Shu-Osher form; ghost fill and state        */
@@ -1641,7 +1668,10 @@ void or_step(or_state *s, double dt)
     } else {
         or_advance_rk3(s);
     }
-    or_chemistry(s);
+    if (s->reaction_type == 0)
+        or_chemistry_fixed(s);
+    else
+        or_chemistry(s);
     s->time += dt;
     s->iteration += 1;
 }

@@ -130,3 +130,33 @@ def test_restart_reader_roundtrip(tmp_path):
         other.restart(pa)
     with pytest.raises(eu3d.EU3DError):
         other.restart(str(tmp_path / "missing.bin"))
+
+
+def test_tecplot_series_output(tmp_path):
+    """Euler::Output's Tecplot file: header, ghost-inclusive block layout
+    (i fastest), 16 variables, the reference's default stream formatting
+    (6 significant digits), values of the host state; load logs written."""
+    ctrl = cases.rsbi(str(tmp_path / "c"))
+    s = eu3d.Solver(ctrl, device=False)
+    out = tmp_path / "out"
+    s.set_output(str(out))
+    s.output(0)
+    f = out / "output_1" / "results" / "SeriesTrapezoid-Adaptive0_0.dat"
+    lines = f.read_text().splitlines()
+    assert lines[0] == "Variables = X,Y,Z,T,D,P,U,V,Ma,Y1,Y2,Y3,Y4,Y5,Y6,Y7"
+    assert lines[1] == f"ZONE I={s.NI}\tJ={s.NJ}\tK={s.NK}"
+    assert lines[2] == "datapacking=block"
+    npts = s.NI * s.NJ * s.NK
+    vals = lines[3:]
+    assert len(vals) == 16 * npts
+    blocks = np.array([float(v) for v in vals]).reshape(16, s.NK, s.NJ, s.NI)
+    x = s.field("xnode")
+    assert vals[1] == "%g" % x[1]                        # the stream's default = %g
+    assert np.allclose(blocks[0][0, 0], x, rtol=1e-5, atol=0)
+    for b, name in [(3, "T"), (4, "D"), (5, "P"), (6, "U"), (7, "V")]:
+        ref = np.transpose(s.field(name), (2, 1, 0))      # (k, j, i)
+        assert np.allclose(blocks[b], ref, rtol=6e-6, atol=1e-300), name
+    y0 = np.transpose(s.field("Yi")[..., 0], (2, 1, 0))
+    assert np.allclose(blocks[9], y0, rtol=6e-6, atol=1e-300)
+    assert (out / "output_1" / "load" / "Load_Loop.dat").exists()
+    assert (out / "output_1" / "load" / "LoadDegree.dat").exists()

@@ -305,6 +305,45 @@ def test_health_flags_nonfinite_and_invalid_cells(tmp_path):
         s.run(3)
 
 
+def test_rsbi_fixed_trapezoid(tmp_path):
+    """ctrl "reaction Trapezoid" (Euler::Trapezoid(type 0)): one sub-step of
+    the full dt per cell, against the oracle's or_chemistry_fixed."""
+    note("rsbi fixed trapezoid 10 steps")
+    ctrl = cases.rsbi(str(tmp_path), steps=10, T_pre=1200.0, dt=1e-7, reaction="Trapezoid")
+    o, s = pair(ctrl)
+    o.run(1)
+    s.run(1)
+    s.sync()
+    compare_state(o, s, 1e-12, "fixed trapezoid 1 step")
+    o.run(9)
+    s.run(9)
+    s.sync()
+    compare_state(o, s, 1e-8, "fixed trapezoid 10 steps")
+    assert np.all(s.interior(s.nchem()) == 1)
+
+
+def test_scheduled_output_and_load_loop(tmp_path):
+    """A run with the reference's output tree: the first Series file at
+    t = 0, the final Output(count), and Load_Loop.dat holding each step's max
+    Nchem, equal to the oracle's."""
+    ctrl = cases.rsbi(str(tmp_path), steps=4, T_pre=1200.0, dt=3e-7)
+    o, s = pair(ctrl)
+    out = tmp_path / "out"
+    s.set_output(str(out))
+    s.run(4)
+    s.output(-1)
+    res = out / "output_1" / "results"
+    assert (res / "SeriesTrapezoid-Adaptive0_0.dat").exists()
+    assert (res / "SeriesTrapezoid-Adaptive1_0.dat").exists()
+    want = []
+    for _ in range(4):
+        o.run(1)
+        want.append(int(o.interior(o.nchem()).max()))
+    rows = [l.split("\t") for l in (out / "output_1" / "load" / "Load_Loop.dat").read_text().splitlines()]
+    assert [int(r[0]) for r in rows] == [1, 2, 3, 4]
+    assert [int(r[1]) for r in rows] == want and max(want) > 1
+
+
 def test_generated_mechanism_equals_runtime_tables(tmp_path):
     """The compile-time mechanism kernels and the run-time-table kernels give
     the same bits (stiff RSBI, 6 steps)."""
