@@ -23,6 +23,7 @@ EU3D_ERR_STATE = -3
 EU3D_ERR_HEALTH = -4
 SCHEME_EE = 0
 SCHEME_TVD_RK3 = 1
+SCHEME_IMEX = 2
 CHEM_COMPACT = 0x1
 CHEM_SKIP = 0x2
 CHEM_KEEP_CMS = 0x4
@@ -97,6 +98,7 @@ SIGNATURES = {
     "eu3d_advance_rk3": (_i, [_vp, _d]),
     "eu3d_chemistry": (_i, [_vp, _d, _i]),
     "eu3d_step": (_i, [_vp, _i, _d, _i]),
+    "eu3d_imex_step": (_i, [_vp, _d, _i]),
     "eu3d_run": (_i, [_vp, _i, _d, _d, _d, _i, _i, ctypes.POINTER(_d), ctypes.POINTER(_i)]),
     "eu3d_health": (_i, [_vp, ctypes.POINTER(Health)]),
     "eu3d_synchronize": (_i, [_vp]),

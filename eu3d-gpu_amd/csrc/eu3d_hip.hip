@@ -368,7 +368,7 @@ __global__ void __launch_bounds__(kBlock) k_ghost_z(Grid g, int nprim, double *p
 
 // ---------------------------------------------------------------------------
 // Fused flux + conservative update.  MODE 0: EE (CS += RHS); 1..3: SSP-RK3
-// stages (1 also saves CS into CS0).
+// stages (1 also saves CS into CS0); 4: the EE right-hand side into CS0 (IMEX).
 //
 // A block owns kTJ rows (j) of kTK = 64 cells (k, one wave per row) and
 // marches over kMarch planes (i); it has kTJ + 1 waves.  Every face flux is
@@ -501,6 +501,10 @@ __global__ void __launch_bounds__(kFluxThreads) __attribute__((amdgpu_waves_per_
 #pragma unroll
             for (int q = 0; q < NV; q++) {
                 double rq = rhs[q] - cz * (face[q * NZF + r * (kTK + 1) + l + 1] - face[q * NZF + r * (kTK + 1) + l]);
+                if (MODE == 4) { // IMEX: the right-hand side alone, into CS0
+                    CS0[q * N + c] = rq;
+                    continue;
+                }
                 double u = CS[q * N + c];
                 double v;
                 if (MODE == 0) {
@@ -999,6 +1003,143 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(EU3
     }
 }
 
+// Euler::IMEX (src/Euler.cpp:1448-1458) after the EE right-hand side was
+// written to RHS: GetPartial_T of the state before the step
+// (src/Flowfield.cpp:2143-2160), Reaction::Diagonalized's rates and Jacobian
+// diagonal (src/Reaction.cpp:1008-1083) and Flowfield::Update_IMEX
+// (src/Flowfield.cpp:1548-1567), then Update_after_CS.  The reference never
+// sets the momentum and energy diagonal entries (0) and reads the energy one
+// past the end of its (NS+3) x (NS+3) matrix; 0 is used for all four.
+template <int NS, class MT>
+__device__ __forceinline__ void imex_term(const MT &m, int r, double T, const double (&Gi)[NS],
+                                          const double (&Mc)[NS], const double (&Di)[NS], const double (&PT)[NS],
+                                          double (&Wr)[NS], double (&MD)[NS])
+{
+    const int nr = m.nr;
+    const double R = m.R, Ru = m.Ru;
+    double KF = m.af[r] * pow_small(T, m.bf[r]);
+    if (m.eaf[r] != 0.0)
+        KF = KF * exp(-m.eaf[r] / (Ru * T));
+    double delta_g = 0.0, RRF = 1.0, RRB = 1.0, RTB = 0.0;
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+        const double nb = m.nb[s * nr + r], nf = m.nf[s * nr + r], tb = m.tb[s * nr + r];
+        if (nb != nf)
+            delta_g += (nb - nf) * Gi[s];
+        if (nf != 0.0)
+            RRF *= pow_small(Mc[s], nf);
+        if (nb != 0.0)
+            RRB *= pow_small(Mc[s], nb);
+        if (tb != 0.0)
+            RTB += Mc[s] * tb;
+    }
+    double Kp = exp(-delta_g / (R * T)) * m.p0x[r];
+    double Kc = Kp * pow_small(R * T, m.mxr[r]);
+    double KB = KF / Kc;
+    if (RTB == 0.0)
+        RTB = 1.0;
+    double RR = KF * RRF - KB * RRB;
+    const double WJH2 = RR * (m.bf[r] / T + m.eaf[r] / (Ru * (T * T)));
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+        const double nb = m.nb[s * nr + r], nf = m.nf[s * nr + r];
+        if (nb != nf) {
+            Wr[s] += (nb - nf) * RTB * RR;
+            double WJH1;
+            if (Di[s] == 0)
+                WJH1 = 0.0;
+            else
+                WJH1 = KF * nf * RRF / Di[s] - KB * nb * RRB / Di[s];
+            MD[s] += (nb - nf) * ((WJH1 + WJH2 * PT[s]) * RTB + RR * m.tb[s * nr + r] / m.mw[s]);
+        }
+    }
+}
+
+template <int NS, class MT>
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(EU3D_CHEM_WAVES_PER_EU)))
+    k_imex(Grid g, const DevMech *__restrict__ mp, double dt, double *__restrict__ CS, const double *__restrict__ RHS,
+           double *__restrict__ prim, double *__restrict__ T, double *__restrict__ E, double *__restrict__ C,
+           double *__restrict__ CMS, int keep_cms)
+{
+    long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    int i, j, k;
+    if (!interior_cell(g, t, i, j, k))
+        return;
+    const auto &m = mech_ref<MT>(mp);
+    const long long N = g.N, c = gidx(g, i, j, k);
+    const double Tc = T[c], D = prim[N + c], U = prim[2 * N + c], V = prim[3 * N + c], W = prim[4 * N + c];
+    double Y[NS], sYM = 0.0;
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+        Y[s] = prim[(7 + s) * N + c];
+        sYM += Y[s] / m.mw[s];
+    }
+    const double Wav = 1.0 / sYM, Rgas = m.R * 1000 / Wav;
+    double PT[NS], Gi[NS];
+    {
+        TPow tp = tpow(Tc);
+        double scp = 0.0, hs[NS];
+        by_range(m, Tc, [&](const auto &a) {
+#pragma unroll
+            for (int s = 0; s < NS; s++) {
+                scp += getcpi(tp, m.ri[s], a, s, NS) * Y[s];
+                hs[s] = gethi(tp, m.ri[s], a, s, NS);
+                Gi[s] = gethi(tp, m.R, a, s, NS) - getsi(tp, m.R, a, s, NS) * Tc;
+            }
+        });
+        const double CV = scp * 1000 - Rgas;
+        const double ke = 0.5 * ((U * U) + (V * V) + (W * W));
+#pragma unroll
+        for (int s = 0; s < NS; s++)
+            PT[s] = (ke - hs[s] + m.ri[s] * 1e3 * Tc) / (CV * D);
+    }
+    double Mc[NS], Di[NS], Wr[NS], MD[NS];
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+        const double cs = CS[s * N + c];
+        Mc[s] = cs / m.mw[s] * 1000 * 1e-6;
+        Di[s] = cs * 1e-3;
+        Wr[s] = 0.0;
+        MD[s] = 0.0;
+    }
+    if constexpr (MT::kGenerated) {
+#pragma unroll
+        for (int r = 0; r < MT::nr; r++)
+            imex_term<NS>(m, r, Tc, Gi, Mc, Di, PT, Wr, MD);
+    } else {
+#pragma unroll 1
+        for (int r = 0; r < m.nr; r++)
+            imex_term<NS>(m, r, Tc, Gi, Mc, Di, PT, Wr, MD);
+    }
+    double cs[NS + 4];
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+        const double cms = m.mw[s] * Wr[s] * 1e3;
+        const double md = m.mw[s] * MD[s];
+        if (keep_cms)
+            CMS[s * N + c] = cms;
+        const double rq = RHS[s * N + c] + cms * dt;
+        cs[s] = CS[s * N + c] + rq / (1.0 - dt * md);
+    }
+#pragma unroll
+    for (int q = NS; q < NS + 4; q++)
+        cs[q] = CS[q * N + c] + RHS[q * N + c] / (1.0 - dt * 0.0);
+#pragma unroll
+    for (int q = 0; q < NS + 4; q++)
+        CS[q * N + c] = cs[q];
+    CellState<NS> st;
+    recover<NS>(m, cs, Tc, E[c], st);
+    store_state<NS>(N, c, st, prim, T, E, C);
+}
+
+template <int NS, class MT>
+static void launch_imex(eu3d_ctx *ctx, double dt, int keep)
+{
+    k_imex<NS, MT><<<blocks_for(ctx->n_int), kBlock, 0, ctx->stream>>>(ctx->g, ctx->mech, dt, ctx->CS, ctx->CS0,
+                                                                       ctx->prim, ctx->T, ctx->E, ctx->C, ctx->CMS,
+                                                                       keep);
+}
+
 // Euler::Trapezoid(type 0): one trapezoid sub-step of the full dt in every
 // interior cell with the rates at its state, no prediction
 // (src/Reaction.cpp:485-559), then the state rebuilt and recovered.  The
@@ -1436,8 +1577,12 @@ static void launch_flux(eu3d_ctx *ctx, int mode, double dt)
         k_flux_tile<NS, 2><<<grid, kFluxThreads, 0, ctx->stream>>>(g, ctx->prim, ctx->CS, ctx->CS0, ctx->x, ctx->y,
                                                                     ctx->z, dt);
         break;
-    default:
+    case 3:
         k_flux_tile<NS, 3><<<grid, kFluxThreads, 0, ctx->stream>>>(g, ctx->prim, ctx->CS, ctx->CS0, ctx->x, ctx->y,
+                                                                    ctx->z, dt);
+        break;
+    default:
+        k_flux_tile<NS, 4><<<grid, kFluxThreads, 0, ctx->stream>>>(g, ctx->prim, ctx->CS, ctx->CS0, ctx->x, ctx->y,
                                                                     ctx->z, dt);
         break;
     }
@@ -2260,10 +2405,37 @@ int eu3d_step(eu3d_ctx *ctx, int scheme, double dt, int chem_flags)
     } else if (scheme == EU3D_SCHEME_TVD_RK3) {
         if ((rc = eu3d_advance_rk3(ctx, dt)))
             return rc;
+    } else if (scheme == EU3D_SCHEME_IMEX) {
+        return eu3d_imex_step(ctx, dt, chem_flags);
     } else {
         FAIL(EU3D_ERR_ARG, "unknown time-advance scheme");
     }
     return eu3d_chemistry(ctx, dt, chem_flags);
+}
+
+int eu3d_imex_step(eu3d_ctx *ctx, double dt, int chem_flags)
+{
+    if (!ctx)
+        return EU3D_ERR_ARG;
+    if (!ctx->has_state)
+        FAIL(EU3D_ERR_STATE, "no state uploaded");
+    if (!(dt > 0))
+        FAIL(EU3D_ERR_ARG, "dt must be > 0");
+    int rc;
+    if ((rc = eu3d_fill_ghosts(ctx)))
+        return rc;
+    {
+        Timer tm(ctx, K_FLUX);
+        EU3D_DISPATCH_NS(ctx->ns, launch_flux, ctx, 4, dt);
+    }
+    {
+        Timer tm(ctx, K_CHEM);
+        k_chem_reset<<<1, 1, 0, ctx->stream>>>(ctx->cnt);
+        EU3D_DISPATCH_NS(ctx->ns, launch_imex, ctx, dt, (chem_flags & EU3D_CHEM_KEEP_CMS) ? 1 : 0);
+        k_chem_health<<<blocks_for(ctx->n_int), kBlock, 0, ctx->stream>>>(ctx->g, ctx->prim, ctx->T, ctx->cnt);
+    }
+    CK(hipGetLastError());
+    return EU3D_OK;
 }
 
 int eu3d_run(eu3d_ctx *ctx, int scheme, double cfl, double dt_fixed, double final_time, int max_steps,

@@ -774,8 +774,10 @@ Euler::Euler(const std::string &ctrlfile, const std::string &base_dir, bool devi
         Reaction_type = 1;
     else if (rs == "Trapezoid")
         Reaction_type = 0;
+    else if (rs == "IMEX")
+        Reaction_type = 2;
     else
-        throw Error(EU3D_ERR_ARG, "reaction scheme must be Trapezoid or Trapezoid-Adaptive");
+        throw Error(EU3D_ERR_ARG, "reaction scheme must be Trapezoid, Trapezoid-Adaptive or IMEX");
     if (Ctrl.getStringParameter("difference") != "MUSCL_2")
         throw Error(EU3D_ERR_ARG, "difference scheme must be MUSCL_2");
     std::string ta = Ctrl.getStringParameter("time-adv");
@@ -785,6 +787,8 @@ Euler::Euler(const std::string &ctrlfile, const std::string &base_dir, bool devi
         TimeAdv_Sch = EU3D_SCHEME_TVD_RK3;
     else
         throw Error(EU3D_ERR_ARG, "time-adv must be EE or TVD_RK3");
+    if (Reaction_type == 2 && TimeAdv_Sch != EU3D_SCHEME_EE)
+        throw Error(EU3D_ERR_ARG, "IMEX needs time-adv EE (it adds the EE right-hand side itself)");
     Mymesh.MeshProcess(path(Ctrl.getStringParameter("gridname")));
     TwoDim.InputRead(path(Ctrl.getStringParameter("initialization")), Mymesh);
     React.ReactionRead(path(Ctrl.getStringParameter("reaction-model")), path(Ctrl.getStringParameter("Thermofile")));
@@ -839,7 +843,8 @@ void Euler::Computing(int max_steps)
             out_num = out_num + 1;
         }
         const int flags = chem_flags | (Reaction_type == 0 ? EU3D_CHEM_FIXED : 0);
-        check(eu3d_step(TwoDim.ctx, TimeAdv_Sch, TwoDim.dt, flags), TwoDim.ctx, "eu3d_step");
+        const int scheme = Reaction_type == 2 ? EU3D_SCHEME_IMEX : TimeAdv_Sch;
+        check(eu3d_step(TwoDim.ctx, scheme, TwoDim.dt, flags), TwoDim.ctx, "eu3d_step");
         time += TwoDim.dt;
         iteration += 1;
         n++;

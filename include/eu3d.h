@@ -34,6 +34,7 @@ extern "C" {
 
 #define EU3D_SCHEME_EE 0      /* Global::EE       (include/Global.hpp) */
 #define EU3D_SCHEME_TVD_RK3 1 /* Global::TVD_RK3  (a stub in the reference, built here) */
+#define EU3D_SCHEME_IMEX 2    /* EE advection with the IMEX source (ctrl "reaction IMEX", see eu3d_imex_step) */
 
 #define EU3D_CHEM_COMPACT 0x1   /* stiff cells (Nchem > 1) are binned into a dense work queue */
 #define EU3D_CHEM_SKIP 0x2      /* advection only (no chemistry) */
@@ -141,6 +142,15 @@ int eu3d_advance_rk3(eu3d_ctx *ctx, double dt);
  * Explicit/Update_after_CS after each (src/Euler.cpp:461-493);
  * reaction_cells_host (src/ReactionCuda.cu:2275-2627). flags: EU3D_CHEM_*. */
 int eu3d_chemistry(eu3d_ctx *ctx, double dt, int flags);
+
+/* Euler::IMEX (src/Euler.cpp:1448-1458) with its EE advection, one step:
+ * ghosts, the EE right-hand side (not added), then per cell GetPartial_T
+ * (src/Flowfield.cpp:2143-2160), Reaction::Diagonalized
+ * (src/Reaction.cpp:1008-1083) and Flowfield::Update_IMEX
+ * (src/Flowfield.cpp:1548-1567): CS += (RHS + W dt) / (1 - dt J_ss) for the
+ * species and CS += RHS for momentum and energy, then Update_after_CS.
+ * Nchem is not changed.  chem_flags: EU3D_CHEM_KEEP_CMS. */
+int eu3d_imex_step(eu3d_ctx *ctx, double dt, int chem_flags);
 
 /* One iteration of Euler::Computing's loop body (src/Euler.cpp:322-389)
  * with the given dt: ghosts, advection (EE or RK3), recovery, chemistry. */

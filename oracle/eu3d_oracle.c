@@ -889,8 +889,10 @@ or_state *or_init(const char *ctrl_path, const char *base_dir, char *err, int er
             s->reaction_type = 1;
         else if (r && strcmp(r, "Trapezoid") == 0)
             s->reaction_type = 0;
+        else if (r && strcmp(r, "IMEX") == 0)
+            s->reaction_type = 2;
         else {
-            snprintf(err, errlen, "reaction scheme must be Trapezoid or Trapezoid-Adaptive");
+            snprintf(err, errlen, "reaction scheme must be Trapezoid, Trapezoid-Adaptive or IMEX");
             goto fail;
         }
         if (!d || strcmp(d, "MUSCL_2") != 0) {
@@ -906,6 +908,10 @@ or_state *or_init(const char *ctrl_path, const char *base_dir, char *err, int er
             goto fail;
         }
         s->diff = 1;
+        if (s->reaction_type == 2 && s->time_adv != 0) {
+            snprintf(err, errlen, "IMEX needs time-adv EE (it adds the EE right-hand side itself)");
+            goto fail;
+        }
     }
     const char *gname = kf_get(ctrl, "gridname"), *iname = kf_get(ctrl, "initialization"),
                *mname = kf_get(ctrl, "reaction-model"), *tname = kf_get(ctrl, "Thermofile");
@@ -1620,6 +1626,69 @@ void or_chemistry_fixed(or_state *s)
             }
 }
 
+/* IMEX (Euler::IMEX, src/Euler.cpp:1448-1458), after Advection has formed
+ * the EE right-hand side (not added to CS).  Per cell, at the state before
+ * the step:
+ *  - GetPartial_T (src/Flowfield.cpp:2143-2160): PT_s = (KE - h_s + R_s 1e3 T)
+ *    / ((Cp - Rgas) D), h_s from GetHi with the species gas constant (kJ/kg,
+ *    as the reference has it);
+ *  - Reaction::Diagonalized (src/Reaction.cpp:1008-1083): rates, CMS = W 1e3,
+ *    MD_s = Mw_s sum_r (nu''-nu') ((WJH1 + WJH2 PT_s) R_TB + RR TB_s / Mw_s);
+ *  - Flowfield::Update_IMEX (src/Flowfield.cpp:1548-1567): RHS_s += CMS_s dt,
+ *    CS += RHS / (1 - dt MD) for the species; the momentum and energy
+ *    diagonal entries are never set in the reference (0), and the energy one
+ *    is read one entry past the (NS+3) x (NS+3) matrix -- 0 is used for all
+ *    four; then Update_after_CS.  Nchem is not touched. */
+void or_imex(or_state *s)
+{
+    int ns = s->ns, nr = s->nr, bc = s->bc, nv = ns + 4;
+    double dt = s->dt, Ru = s->Ru;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static)
+#endif
+    for (int i = bc; i < s->ni + bc; i++)
+        for (int j = bc; j < s->nj + bc; j++)
+            for (int k = bc; k < s->nk + bc; k++) {
+                size_t id = IDX(s, i, j, k);
+                double T = s->T[id], D = s->D[id], U = s->U[id], V = s->V[id], W = s->W[id];
+                double CV = s->Cp[id] - s->Rgas[id];
+                double PT[OR_MAX_NS], Mc[OR_MAX_NS], Di[OR_MAX_NS], MD[OR_MAX_NS];
+                double KF[OR_MAX_NR], KB[OR_MAX_NR], RRF[OR_MAX_NR], RRB[OR_MAX_NR], RTB[OR_MAX_NR],
+                    RR[OR_MAX_NR], Wi[OR_MAX_NS];
+                for (int q = 0; q < ns; q++) {
+                    double Hi = or_gethi(T, s->ri[q], q, s->coeff0, s->coeff1, ns);
+                    PT[q] = (0.5 * ((U * U) + (V * V) + (W * W)) - Hi + s->ri[q] * 1e3 * T) / (CV * D);
+                    Di[q] = s->Di[id * ns + q] * 1e-3;
+                    Mc[q] = s->Mc[id * ns + q] * 1e-6;
+                }
+                or_rates(s, T, Mc, KF, KB, RRF, RRB, RTB, RR, Wi);
+                for (int q = 0; q < ns; q++) {
+                    double temp = 0.0;
+                    for (int r = 0; r < nr; r++) {
+                        double nb = s->stoi_b[(size_t)q * nr + r], nf = s->stoi_f[(size_t)q * nr + r];
+                        double WJH1;
+                        if (Di[q] == 0)
+                            WJH1 = 0.0;
+                        else
+                            WJH1 = KF[r] * nf * RRF[r] / Di[q] - KB[r] * nb * RRB[r] / Di[q];
+                        double WJH2 = RR[r] * (s->bf[r] / T + s->eaf[r] / (Ru * (T * T)));
+                        temp += (nb - nf) * ((WJH1 + WJH2 * PT[q]) * RTB[r] +
+                                             RR[r] * s->react_tb[(size_t)q * nr + r] / s->mw[q]);
+                    }
+                    MD[q] = s->mw[q] * temp;
+                }
+                double *cs = s->CS + id * nv, *rhs = s->RHS + id * nv;
+                for (int q = 0; q < ns; q++) {
+                    s->CMS[id * ns + q] = Wi[q] * 1e3;
+                    double rq = rhs[q] + s->CMS[id * ns + q] * dt;
+                    cs[q] = cs[q] + rq / (1.0 - dt * MD[q]);
+                }
+                for (int q = ns; q < nv; q++)
+                    cs[q] = cs[q] + rhs[q] / (1.0 - dt * 0.0);
+                update_after_cs_cell(s, id);
+            }
+}
+
 /* ====================================================================== */
 /* SSP-RK3 advection (new; TimeAdv::TVD_RK3 is a no-op in the reference,  */
 /* src/TimeAdv.cpp:210-221).  Shu-Osher form; ghost fill and state        */
@@ -1661,6 +1730,14 @@ void or_advance_rk3(or_state *s)
 void or_step(or_state *s, double dt)
 {
     s->dt = dt;
+    if (s->reaction_type == 2) { /* IMEX: EE right-hand side + diagonal implicit source */
+        or_field_boundary(s);
+        or_advection(s);
+        or_imex(s);
+        s->time += dt;
+        s->iteration += 1;
+        return;
+    }
     if (s->time_adv == 0) {
         or_field_boundary(s);
         or_advection(s);

@@ -54,7 +54,7 @@ typedef struct or_state {
     int count;
     int time_adv;                /* 0 = EE, 1 = TVD_RK3 */
     int diff;                    /* 1 = MUSCL_2 (only supported scheme) */
-    int reaction_type;           /* 0 = Trapezoid, 1 = Trapezoid-Adaptive */
+    int reaction_type;           /* 0 = Trapezoid, 1 = Trapezoid-Adaptive, 2 = IMEX */
 
     /* flow field (include/Flowfield.hpp:68-110) */
     double *U, *V, *W, *P, *D, *T, *C, *Ma, *Wav, *Rgas, *Cp, *H, *E, *Gamma;
@@ -88,6 +88,7 @@ void or_update_after_adv(or_state *s);                            /* CS += RHS; 
 void or_update_after_cs(or_state *s);                             /* Update_after_CS() */
 void or_chemistry(or_state *s);                                   /* Euler::Trapezoid(1), CPU branch */
 void or_chemistry_fixed(or_state *s);                             /* Euler::Trapezoid(0) */
+void or_imex(or_state *s);                                        /* Euler::IMEX, after or_advection */
 void or_prediction(or_state *s);                                  /* TrapezoidPrediction only */
 void or_advance_rk3(or_state *s);                                 /* SSP-RK3 advection (new) */
 void or_step(or_state *s, double dt);                             /* one Computing iteration with given dt */

@@ -20,6 +20,8 @@ _SIGS = {
     "or_update_after_adv": (None, [ctypes.c_void_p]),
     "or_update_after_cs": (None, [ctypes.c_void_p]),
     "or_chemistry": (None, [ctypes.c_void_p]),
+    "or_chemistry_fixed": (None, [ctypes.c_void_p]),
+    "or_imex": (None, [ctypes.c_void_p]),
     "or_prediction": (None, [ctypes.c_void_p]),
     "or_advance_rk3": (None, [ctypes.c_void_p]),
     "or_step": (None, [ctypes.c_void_p, ctypes.c_double]),
@@ -166,6 +168,13 @@ class Oracle:
 
     def prediction(self):
         self.L.or_prediction(self.h)
+
+    def chemistry_fixed(self):
+        self.L.or_chemistry_fixed(self.h)
+
+    def imex(self):
+        """IMEX source update after advection() (or_imex)."""
+        self.L.or_imex(self.h)
 
     def advance_rk3(self):
         self.L.or_advance_rk3(self.h)

@@ -24,7 +24,7 @@ M_wave          2.0
 NS              9
 NR              19
 bubble_xc_frac  0.35
-bubble_r_frac   0.2
+bubble_r_frac   {r_frac}
 shock_gap_cells {gap}
 tanh_cells      2
 perturb_amp     0.01
@@ -87,11 +87,14 @@ def rsbi(tmpdir, ni=17, nj=13, nk=4, steps=3, time_adv="EE", T_pre=1000.0, M_wav
                       reaction=reaction)
 
 
-def sbi(tmpdir, ni=32, nj=16, nk=16, h=1e-4, gap=3, seed=20240601, time_adv="TVD_RK3", cfl=0.5):
-    """Spherical shock-bubble (3dSBI) on a reduced grid with spacing h."""
+def sbi(tmpdir, ni=32, nj=16, nk=16, h=1e-4, gap=3, seed=20240601, time_adv="TVD_RK3", cfl=0.5, r_frac=0.2,
+        reaction="Trapezoid-Adaptive"):
+    """Spherical shock-bubble (3dSBI) on a reduced grid with spacing h
+    (a large negative r_frac leaves no bubble gas: pure nitrogen)."""
     return write_case(tmpdir, f"sbi{ni}x{nj}x{nk}", ni, nj, nk,
                       (0.0, (ni - 1) * h, 0.0, (nj - 1) * h, 0.0, (nk - 1) * h),
-                      SBI_INPUT.format(gap=gap, seed=seed), time_adv=time_adv, cfl=cfl, dt=0.0)
+                      SBI_INPUT.format(gap=gap, seed=seed, r_frac=r_frac), time_adv=time_adv, cfl=cfl, dt=0.0,
+                      reaction=reaction)
 
 
 def znd(tmpdir, ni=201, time_adv="TVD_RK3", cfl=0.5, length=0.01):

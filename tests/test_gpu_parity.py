@@ -344,6 +344,23 @@ def test_scheduled_output_and_load_loop(tmp_path):
     assert [int(r[1]) for r in rows] == want and max(want) > 1
 
 
+def test_rsbi_imex(tmp_path):
+    """ctrl "reaction IMEX" (Euler::IMEX with EE advection): against the
+    oracle's or_imex, 1 and 10 steps."""
+    note("rsbi imex 10 steps")
+    ctrl = cases.rsbi(str(tmp_path), steps=10, T_pre=1200.0, dt=3e-8, reaction="IMEX")
+    o, s = pair(ctrl)
+    o.run(1)
+    s.run(1)
+    s.sync()
+    compare_state(o, s, 1e-12, "IMEX 1 step")
+    assert rel_linf(s.interior(s.field("CMS")), o.interior(o.field("CMS"))) <= 1e-12
+    o.run(9)
+    s.run(9)
+    s.sync()
+    compare_state(o, s, 1e-8, "IMEX 10 steps")
+
+
 def test_generated_mechanism_equals_runtime_tables(tmp_path):
     """The compile-time mechanism kernels and the run-time-table kernels give
     the same bits (stiff RSBI, 6 steps)."""

@@ -372,3 +372,40 @@ def test_mlp_oracle_gradients_match_central_differences():
         fd = (mo.loss_and_grads(pp, H, x, t, norm, n_in, n_out, emulate=False)[0] -
               mo.loss_and_grads(pm, H, x, t, norm, n_in, n_out, emulate=False)[0]) / (2 * h)
         assert abs(g[i] - fd) <= 1e-6 * max(1.0, abs(fd)) + 1e-9, (i, g[i], fd)
+
+
+# ---------------------------------------------------------------------------
+# alternative chemistry schemes
+# ---------------------------------------------------------------------------
+def _bits(a):
+    return np.ascontiguousarray(a).view(np.int64)
+
+
+def test_fixed_trapezoid_equals_adaptive_when_nchem_is_one(tmp_path):
+    """Where the adaptive prediction gives Nchem = 1 everywhere, the
+    non-adaptive Trapezoid (one sub-step of dt) is the same computation."""
+    a = oracle.Oracle(cases.rsbi(str(tmp_path / "a"), dt=1e-9))
+    b = oracle.Oracle(cases.rsbi(str(tmp_path / "b"), dt=1e-9, reaction="Trapezoid"))
+    a.run(2)
+    b.run(2)
+    assert np.all(a.interior(a.nchem()) == 1)
+    for n in ["CS", "T", "P", "Yi", "CMS"]:
+        assert np.array_equal(_bits(a.interior(a.field(n))), _bits(b.interior(b.field(n)))), n
+
+
+def test_imex_is_ee_update_for_an_inert_mixture(tmp_path):
+    """Pure nitrogen has no reaction (every rate of progress is 0, so W = 0
+    and the Jacobian diagonal is 0): the IMEX update is the EE update
+    CS += RHS and the same recovery, bit for bit."""
+    ctrl = cases.sbi(str(tmp_path), ni=16, nj=8, nk=8, r_frac=-100.0, time_adv="EE", cfl=0.5)
+    a, b = oracle.Oracle(ctrl), oracle.Oracle(ctrl)
+    assert np.all(a.interior(a.field("Yi"))[..., 8] == 1.0)
+    for o in (a, b):
+        o.set_dt(o.cfl(0.5, 1.0))
+        o.field_boundary()
+        o.advection()
+    a.update_after_adv()
+    b.imex()
+    for n in ["CS", "T", "P", "D", "U", "Yi"]:
+        assert np.array_equal(_bits(a.interior(a.field(n))), _bits(b.interior(b.field(n)))), n
+    assert np.all(b.interior(b.field("CMS")) == 0.0)
