@@ -208,6 +208,46 @@ def odw_side_by_side(args):
     print(json.dumps(out))
 
 
+# The reference's published main-loop time for its own problem (3dRSBI,
+# 281 x 141 x 16 nodes, MUSCL_2 + EE + Trapezoid-Adaptive, dt = 3e-8 s, 100
+# steps): 9.0994 s on an NVIDIA A800 (docs of the reference).
+REF_RSBI_CELL_STEPS = 633936 * 100 / 9.0994
+
+
+def rsbi_reference_problem(args):
+    """The reference's benchmark problem as it ships (input/ctrl-3dRSBI.txt):
+    the whole 100-step main loop, timed on the host clock."""
+    eu3d = eu3d_pkg()
+    L = eu3d.load()
+    base = os.path.join(REPO, "eu3d-gpu_amd")
+    s = eu3d.Solver(os.path.join(base, "input", "ctrl-3dRSBI.txt"), base_dir=base)
+    n_int = s.ni * s.nj * s.nk
+    flags = eu3d.CHEM_COMPACT | eu3d.CHEM_KEEP_CMS
+    warm = eu3d.Solver(os.path.join(base, "input", "ctrl-3dRSBI.txt"), base_dir=base)
+    warm.run(args.warmup, flags)      # warm-up on a second solver: the timed one starts at t = 0
+    eu3d.check(L.eu3d_synchronize(warm.ctx), warm.ctx)
+    warm.close()
+    eu3d.check(L.eu3d_synchronize(s.ctx), s.ctx)
+    t0 = time.perf_counter()
+    s.run(100, flags)
+    eu3d.check(L.eu3d_synchronize(s.ctx), s.ctx)
+    el = time.perf_counter() - t0
+    info = s.info()
+    rc, h = s.health()
+    steps = info["iteration"]
+    value = n_int * steps / el
+    out = {"metric": METRIC, "value": value, "unit": "cell-steps/s", "n_gpus": 1, "steps": steps,
+           "warmup": args.warmup, "ms_per_step": el / steps * 1e3, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": value / REF_RSBI_CELL_STEPS, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": f"the reference's 3dRSBI problem {s.ni}x{s.nj}x{s.nk}, EE, dt 3e-8 s, "
+                                  f"100 steps (input/ctrl-3dRSBI.txt)", "grid": [s.ni, s.nj, s.nk]},
+           "baseline": {"value": REF_RSBI_CELL_STEPS, "hardware": "NVIDIA A800 80GB PCIe, main loop 9.0994 s"},
+           "chem_health": {"rc": rc, "max_nchem": h.max_nchem, "stiff_cells": h.stiff_cells},
+           "sim_time_end": info["time"]}
+    s.close()
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -218,7 +258,7 @@ def main():
     ap.add_argument("--no-surrogate", action="store_true",
                     help="skip the chemistry-call comparison FP64 vs the committed surrogate")
     ap.add_argument("--cpu-threads", type=int, default=0)
-    ap.add_argument("--workload", choices=["sbi", "odw", "odw-large"], default="sbi",
+    ap.add_argument("--workload", choices=["sbi", "odw", "odw-large", "rsbi"], default="sbi",
                     help="sbi: the default line; odw: oblique detonation, FP64 and surrogate side by side")
     ap.add_argument("--develop", type=int, default=3000, help="odw: steps before the timed region")
     ap.add_argument("--model", default=None, help="odw: surrogate model file (default: the committed one)")
@@ -243,6 +283,9 @@ def main():
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(multi-GPU decomposition is out of scope)", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "rsbi":
+        rsbi_reference_problem(args)
+        return
     if args.workload != "sbi":
         odw_side_by_side(args)
         return
