@@ -283,14 +283,13 @@ __device__ __forceinline__ double muscl_upwind(bool up, double fm, double f0, do
 
 // ---------------------------------------------------------------------------
 // AUSM+ interface flux (src/Flowfield.cpp:1797-1931).  dir 0/1/2 = x/y/z.
-// Field order in prim: P D U V W H Gamma Y0..Y(NS-1), each an N-long array.
-// a = index of the left point, st = stride of the direction.
+// Field order: P D U V W H Gamma Y0..Y(NS-1).  ld(f, o) returns field f at
+// the point o (-1 .. 2) along the face direction, 0 being the left point.
 // ---------------------------------------------------------------------------
-template <int NS>
-__device__ __forceinline__ void face_flux(int dir, const double *__restrict__ prim, long long N, long long a,
-                                          long long st, double (&F)[NS + 4])
+template <int NS, class Ld>
+__device__ __forceinline__ void face_flux_ld(int dir, const Ld &ld, double (&F)[NS + 4])
 {
-#define LD(f, o) prim[(long long)(f) * N + a + (o) * st]
+#define LD(f, o) ld(f, o)
     double PL = muscl_left(LD(0, -1), LD(0, 0), LD(0, 1));
     double PR = muscl_right(LD(0, 0), LD(0, 1), LD(0, 2));
     double DL = muscl_left(LD(1, -1), LD(1, 0), LD(1, 1));
@@ -349,6 +348,21 @@ __device__ __forceinline__ void face_flux(int dir, const double *__restrict__ pr
     F[NS + dir] = F[NS + dir] + G1 * PL + G2 * PR;
     F[NS + 3] = VI * (Dup * Hup + 0.5 * Dup * ((vel[0] * vel[0]) + (vel[1] * vel[1]) + (vel[2] * vel[2])));
 #undef LD
+}
+
+// Stencil points from the SoA arrays in HBM: prim[f N + a + o st], a = index
+// of the left point, st = stride of the direction.
+struct GlobalLd {
+    const double *__restrict__ prim;
+    long long N, a, st;
+    __device__ __forceinline__ double operator()(int f, int o) const { return prim[(long long)f * N + a + o * st]; }
+};
+
+template <int NS>
+__device__ __forceinline__ void face_flux(int dir, const double *__restrict__ prim, long long N, long long a,
+                                          long long st, double (&F)[NS + 4])
+{
+    face_flux_ld<NS>(dir, GlobalLd{prim, N, a, st}, F);
 }
 
 // ---------------------------------------------------------------------------

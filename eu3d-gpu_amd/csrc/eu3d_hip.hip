@@ -369,35 +369,43 @@ __global__ void __launch_bounds__(kBlock) k_ghost_z(Grid g, int nprim, double *p
 // ---------------------------------------------------------------------------
 // Fused flux + conservative update.  MODE 0: EE (CS += RHS); 1..3: SSP-RK3
 // stages (1 also saves CS into CS0); 4: the EE right-hand side into CS0 (IMEX).
-//
-// A block owns kTJ rows (j) of kTK = 64 cells (k, one wave per row) and
-// marches over kMarch planes (i); it has kTJ + 1 waves.  Every face flux is
-// computed once:
-//   x faces: by the owning thread, the plus face of plane i is kept in
-//            registers as the minus face of plane i+1;
-//   y faces: kTJ+1 rows of 64 faces, one per thread (the extra wave computes
-//            the faces below the tile), exchanged through LDS;
-//   z faces: kTJ rows of 65 faces on the first kTJ*65 threads, through LDS.
 // RHS = -dt/dx (F+ - F-) - dt/dy (G+ - G-) - dt/dz (Q+ - Q-) in that order
 // (TimeAdv::EE, src/TimeAdv.cpp:51-54).
+//
+// A block owns kTJ rows (j) of kTK = 64 cells (k, one wave per row) and
+// marches over kMarch planes (i); it has kTJ + 1 waves.  Per plane it stages
+// a tile of every primitive field -- rows j0-2 .. j0+kTJ+1, columns
+// k0-2 .. k0+kTK+1, ghost layers included -- in LDS once, and the y and z
+// stencils read from there.  Every face flux is computed once:
+//   x faces: by the owning thread from HBM/L2; the +i face of plane i is
+//            kept in registers as the -i face of plane i+1;
+//   y faces: wave w < kTJ computes the face above its row (slot w+1), the
+//            extra wave the face below the tile (slot 0); exchanged in LDS;
+//   z faces: every owning thread computes the face at its +k side; the face
+//            at its -k side is the neighbour lane's (a wave shuffle), for
+//            lane 0 the one the extra wave computes (lanes < kTJ) into LDS.
+// The next plane's tile is loaded after the update, so two barriers per
+// plane suffice; the own cell's conserved state is loaded before the face
+// work so that its latency overlaps it.
 // ---------------------------------------------------------------------------
 // Tile shape: 7 rows + 1 halo wave = 8 waves, which fill the 8 wave slots of
-// a CU at the kernel's register occupancy (2 waves per SIMD); 8 planes per
-// block give about 5 rounds of blocks over the chip at 256 x 128 x 128.
+// a CU at the kernel's register occupancy (2 waves per SIMD); the LDS tile
+// and faces take 146 KiB of the CU's 160 KiB; 8 planes per block give about
+// 5 rounds of blocks over the chip at 256 x 128 x 128.
 #ifndef EU3D_FLUX_TJ
 #define EU3D_FLUX_TJ 7
 #endif
 #ifndef EU3D_FLUX_MARCH
 #define EU3D_FLUX_MARCH 8
 #endif
-#ifndef EU3D_FLUX_WAVES_PER_EU
-#define EU3D_FLUX_WAVES_PER_EU 2
-#endif
 constexpr int kTJ = EU3D_FLUX_TJ;
 constexpr int kTK = 64;
-constexpr int kFluxThreads = 64 * (kTJ + 1);
+constexpr int kFluxWaves = kTJ + 1;
+constexpr int kFluxThreads = 64 * kFluxWaves;
 constexpr int kMarch = EU3D_FLUX_MARCH;
-constexpr int kXCD = 8; // L2 domains (XCDs); workgroups are dealt to them round-robin
+constexpr int kTR = kTJ + 4; // tile rows
+constexpr int kTC = kTK + 4; // tile columns
+constexpr int kXCD = 8;      // L2 domains (XCDs); workgroups are dealt to them round-robin
 
 // Tile of a flux block.  Consecutive workgroup ids land on different XCDs,
 // each with its own L2; the tiles are renumbered so that every XCD gets a
@@ -427,44 +435,122 @@ static unsigned int flux_blocks(const Grid &g)
     return (total + kXCD - 1) / kXCD * kXCD;
 }
 
-template <int NS, int MODE>
-__global__ void __launch_bounds__(kFluxThreads) __attribute__((amdgpu_waves_per_eu(EU3D_FLUX_WAVES_PER_EU)))
-    k_flux_tile(Grid g, const double *__restrict__ prim, double *__restrict__ CS, double *__restrict__ CS0,
-                const double *__restrict__ xn, const double *__restrict__ yn, const double *__restrict__ zn,
-                double dt)
+// Stencil points from the LDS tile: base = row * kTC + column of the left
+// point, st = kTC (y) or 1 (z); fields kTR * kTC apart.
+struct TileLd {
+    const double *tile;
+    int base, st;
+    __device__ __forceinline__ double operator()(int f, int o) const { return tile[f * (kTR * kTC) + base + o * st]; }
+};
+
+// Tile loads of one thread: (field, row) pairs are dealt to the waves, one
+// lane per column of the kTK interior columns (2 .. kTK+1); the 4 edge
+// columns of every pair (0, 1, kTK+2, kTK+3) are dealt to all threads.
+// Points outside the array (rows >= NJ or columns >= NK of a partial tile)
+// read as 0; no face that uses them is applied.  Rows and columns below the
+// tile are in the array: j0, k0 >= bc >= 2.
+template <int NP>
+struct TilePlan {
+    static constexpr int kPairs = NP * kTR;
+    static constexpr int kMain = (kPairs + kFluxWaves - 1) / kFluxWaves;
+    static constexpr int kEdges = kPairs * 4;
+    static constexpr int kEdge = (kEdges + kFluxThreads - 1) / kFluxThreads;
+    static constexpr int kRegs = kMain + kEdge;
+};
+
+template <int NP>
+__device__ __forceinline__ void tile_fetch(const Grid &g, const double *__restrict__ prim, int i, int j0, int k0,
+                                           int w, int l, int t, double (&v)[TilePlan<NP>::kRegs])
 {
-    constexpr int NV = NS + 4;
-    constexpr int NYF = (kTJ + 1) * kTK; // y faces per plane
-    constexpr int NZF = kTJ * (kTK + 1); // z faces per plane
-    static_assert(NZF <= NYF, "LDS face buffer sized for the y faces");
-    __shared__ double face[NV * NYF];
-    FluxTile tile;
-    if (!flux_tile(g, tile))
-        return;
-    const int t = threadIdx.x;
+    using P = TilePlan<NP>;
     const long long N = g.N;
-    const int k0 = tile.k0, j0 = tile.j0, i0 = tile.i0;
+    // formed per plane: strength-reduced across the march, the addresses
+    // would be held in registers for the whole loop
+    long long plane = (long long)i * g.NJ;
+    int lane = l, thr = t;
+    asm volatile("" : "+s"(plane), "+v"(lane), "+v"(thr));
+    // unconditional loads from clamped in-array indices; the value of a
+    // point outside the array is replaced by 0 afterwards
+#pragma unroll
+    for (int m = 0; m < P::kMain; m++) {
+        const int p = min(w + kFluxWaves * m, P::kPairs - 1); // wave-uniform
+        const int f = p / kTR, tr = p - f * kTR;
+        const int j = j0 - 2 + tr, k = k0 + lane;
+        const bool in = j < g.NJ && k < g.NK;
+        const double x = prim[(long long)f * N + (plane + min(j, g.NJ - 1)) * g.NK + min(k, g.NK - 1)];
+        v[m] = in ? x : 0.0;
+    }
+#pragma unroll
+    for (int m = 0; m < P::kEdge; m++) {
+        const int e = min(thr + kFluxThreads * m, P::kEdges - 1);
+        const int p = e >> 2, c = e & 3;
+        const int f = p / kTR, tr = p - f * kTR;
+        const int j = j0 - 2 + tr, k = k0 - 2 + (c < 2 ? c : kTK + c);
+        const bool in = j < g.NJ && k < g.NK;
+        const double x = prim[(long long)f * N + (plane + min(j, g.NJ - 1)) * g.NK + min(k, g.NK - 1)];
+        v[P::kMain + m] = in ? x : 0.0;
+    }
+}
+
+template <int NP>
+__device__ __forceinline__ void tile_store(double *tile, int w, int l, int t, const double (&v)[TilePlan<NP>::kRegs])
+{
+    using P = TilePlan<NP>;
+#pragma unroll
+    for (int m = 0; m < P::kMain; m++) {
+        const int p = w + kFluxWaves * m;
+        if (p < P::kPairs)
+            tile[p * kTC + 2 + l] = v[m];
+    }
+#pragma unroll
+    for (int m = 0; m < P::kEdge; m++) {
+        const int e = t + kFluxThreads * m;
+        const int p = e >> 2, c = e & 3;
+        if (e < P::kEdges)
+            tile[p * kTC + (c < 2 ? c : kTK + c)] = v[P::kMain + m];
+    }
+}
+
+template <int NS, int MODE>
+__global__ void __launch_bounds__(kFluxThreads) __attribute__((amdgpu_waves_per_eu(2)))
+    k_flux_update(Grid g, const double *__restrict__ prim, double *__restrict__ CS, double *__restrict__ CS0,
+                  const double *__restrict__ xn, const double *__restrict__ yn, const double *__restrict__ zn,
+                  double dt)
+{
+    constexpr int NV = NS + 4, NP = NS + 7;
+    using P = TilePlan<NP>;
+    __shared__ double tile[NP * kTR * kTC];
+    __shared__ double fy[NV * kFluxWaves * kTK]; // y faces, slot s between rows j0-1+s and j0+s
+    __shared__ double zb[NV * kTJ];              // z faces between k0-1 and k0, one per row
+    __shared__ double zp0[NV * kTJ];             // z faces between k0 and k0+1, one per row
+    FluxTile tl;
+    if (!flux_tile(g, tl))
+        return;
+    const int t = threadIdx.x, l = t & 63;
+    const int w = __builtin_amdgcn_readfirstlane(t >> 6);
+    const long long N = g.N;
+    const int k0 = tl.k0, j0 = tl.j0, i0 = tl.i0;
     const int i1 = min(i0 + kMarch, g.ni + g.bc);
     const int kend = g.nk + g.bc, jend = g.nj + g.bc;
-    // the cell this thread owns
-    const int r = t >> 6, l = t & 63;
-    const int jc = j0 + r, kc = k0 + l;
-    const bool own = (t < kTJ * kTK) && (jc < jend) && (kc < kend);
-    // the y face this thread computes: between j = yj and yj+1 at k = yk
-    const int yj = j0 - 1 + (t >> 6), yk = k0 + (t & 63);
-    const bool yact = (yj < jend) && (yk < kend);
-    // the z face this thread computes: between k = zk and zk+1 in row zj
-    const int zr = t / (kTK + 1), zf = t - zr * (kTK + 1);
-    const int zj = j0 + zr, zk = k0 - 1 + zf;
-    const bool zact = (t < NZF) && (zj < jend) && (zk < kend);
+    const int jc = j0 + w, kc = k0 + l;
+    const bool own = (w < kTJ) && (jc < jend) && (kc < kend);
+    const int ys = (w < kTJ) ? w + 1 : 0; // y face slot of this thread
+    const bool yact = (j0 - 1 + ys < jend) && (kc < kend);
+    const bool zhalo = (w == kTJ) && (l < kTJ) && (j0 + l < jend);
 
+    double v[P::kRegs];
+    tile_fetch<NP>(g, prim, i0, j0, k0, w, l, t, v);
+    tile_store<NP>(tile, w, l, t, v);
     double rhs[NV], Fxm[NV];
     if (own)
         face_flux<NS>(0, prim, N, gidx(g, i0 - 1, jc, kc), g.sI, Fxm);
     for (int i = i0; i < i1; i++) {
+        const long long c = gidx(g, i, jc, kc);
         if (own) {
             double Fxp[NV];
-            face_flux<NS>(0, prim, N, gidx(g, i, jc, kc), g.sI, Fxp);
+            long long ax = c;
+            asm volatile("" : "+v"(ax)); // addressed per plane, not strength-reduced
+            face_flux<NS>(0, prim, N, ax, g.sI, Fxp);
             double cx = -dt / (xn[i] - xn[i - 1]);
 #pragma unroll
             for (int q = 0; q < NV; q++) {
@@ -472,55 +558,83 @@ __global__ void __launch_bounds__(kFluxThreads) __attribute__((amdgpu_waves_per_
                 Fxm[q] = Fxp[q];
             }
         }
+        double u0[NV];
+        if (own && MODE != 4) {
+#pragma unroll
+            for (int q = 0; q < NV; q++)
+                u0[q] = CS[q * N + c];
+        }
+        __syncthreads(); // tile of plane i complete
+        // LDS addresses are loop-invariant; formed here, not held across the loop
+        int ybase = (ys + 1) * kTC + l + 2, zbase = (w + 2) * kTC + l + 2, hbase = (l + 2) * kTC + 1;
+        asm volatile("" : "+v"(ybase), "+v"(zbase), "+v"(hbase));
         if (yact) {
             double F[NV];
-            face_flux<NS>(1, prim, N, gidx(g, i, yj, yk), g.sJ, F);
+            face_flux_ld<NS>(1, TileLd{tile, ybase, kTC}, F);
 #pragma unroll
             for (int q = 0; q < NV; q++)
-                face[q * NYF + t] = F[q];
+                fy[(q * kFluxWaves + ys) * kTK + l] = F[q];
         }
-        __syncthreads();
+        double Fzp[NV] = {};
+        if (own)
+            face_flux_ld<NS>(2, TileLd{tile, zbase, 1}, Fzp);
+        if (zhalo) {
+            double F[NV];
+            face_flux_ld<NS>(2, TileLd{tile, hbase, 1}, F);
+#pragma unroll
+            for (int q = 0; q < NV; q++)
+                zb[q * kTJ + l] = F[q];
+        }
+        // z differences Q+ - Q- before the barrier: Q- is the neighbour
+        // lane's Q+; lane 0 parks its Q+ in LDS and forms it after the
+        // barrier from the extra wave's face
+        double dz[NV];
+#pragma unroll
+        for (int q = 0; q < NV; q++) {
+            dz[q] = Fzp[q] - __shfl_up(Fzp[q], 1);
+            if (own && l == 0)
+                zp0[q * kTJ + w] = Fzp[q];
+        }
+        // the saved stage state of RK stages 2 and 3, loaded before the
+        // barrier wait
+        double s0[NV];
+        if (own && (MODE == 2 || MODE == 3)) {
+#pragma unroll
+            for (int q = 0; q < NV; q++)
+                s0[q] = CS0[q * N + c];
+        }
+        __syncthreads(); // faces of plane i complete, tile of plane i no longer read
         if (own) {
             double cy = dt / (yn[jc] - yn[jc - 1]);
-#pragma unroll
-            for (int q = 0; q < NV; q++)
-                rhs[q] = rhs[q] - cy * (face[q * NYF + (r + 1) * kTK + l] - face[q * NYF + r * kTK + l]);
-        }
-        __syncthreads();
-        if (zact) {
-            double F[NV];
-            face_flux<NS>(2, prim, N, gidx(g, i, zj, zk), 1, F);
-#pragma unroll
-            for (int q = 0; q < NV; q++)
-                face[q * NZF + t] = F[q];
-        }
-        __syncthreads();
-        if (own) {
             double cz = dt / (zn[kc] - zn[kc - 1]);
-            const long long c = gidx(g, i, jc, kc);
 #pragma unroll
             for (int q = 0; q < NV; q++) {
-                double rq = rhs[q] - cz * (face[q * NZF + r * (kTK + 1) + l + 1] - face[q * NZF + r * (kTK + 1) + l]);
+                double r1 = rhs[q] - cy * (fy[(q * kFluxWaves + w + 1) * kTK + l] - fy[(q * kFluxWaves + w) * kTK + l]);
+                double d = (l == 0) ? zp0[q * kTJ + w] - zb[q * kTJ + w] : dz[q];
+                double rq = r1 - cz * d;
                 if (MODE == 4) { // IMEX: the right-hand side alone, into CS0
                     CS0[q * N + c] = rq;
                     continue;
                 }
-                double u = CS[q * N + c];
-                double v;
+                double u = u0[q];
+                double vv;
                 if (MODE == 0) {
-                    v = u + rq;
+                    vv = u + rq;
                 } else if (MODE == 1) {
                     CS0[q * N + c] = u;
-                    v = u + rq;
+                    vv = u + rq;
                 } else if (MODE == 2) {
-                    v = 0.75 * CS0[q * N + c] + 0.25 * (u + rq);
+                    vv = 0.75 * s0[q] + 0.25 * (u + rq);
                 } else {
-                    v = (1.0 / 3.0) * CS0[q * N + c] + (2.0 / 3.0) * (u + rq);
+                    vv = (1.0 / 3.0) * s0[q] + (2.0 / 3.0) * (u + rq);
                 }
-                CS[q * N + c] = v;
+                CS[q * N + c] = vv;
             }
         }
-        __syncthreads();
+        if (i + 1 < i1) {
+            tile_fetch<NP>(g, prim, i + 1, j0, k0, w, l, t, v);
+            tile_store<NP>(tile, w, l, t, v);
+        }
     }
 }
 
@@ -1566,23 +1680,23 @@ static void launch_flux(eu3d_ctx *ctx, int mode, double dt)
     const dim3 grid(flux_blocks(g));
     switch (mode) {
     case 0:
-        k_flux_tile<NS, 0><<<grid, kFluxThreads, 0, ctx->stream>>>(g, ctx->prim, ctx->CS, ctx->CS0, ctx->x, ctx->y,
+        k_flux_update<NS, 0><<<grid, kFluxThreads, 0, ctx->stream>>>(g, ctx->prim, ctx->CS, ctx->CS0, ctx->x, ctx->y,
                                                                     ctx->z, dt);
         break;
     case 1:
-        k_flux_tile<NS, 1><<<grid, kFluxThreads, 0, ctx->stream>>>(g, ctx->prim, ctx->CS, ctx->CS0, ctx->x, ctx->y,
+        k_flux_update<NS, 1><<<grid, kFluxThreads, 0, ctx->stream>>>(g, ctx->prim, ctx->CS, ctx->CS0, ctx->x, ctx->y,
                                                                     ctx->z, dt);
         break;
     case 2:
-        k_flux_tile<NS, 2><<<grid, kFluxThreads, 0, ctx->stream>>>(g, ctx->prim, ctx->CS, ctx->CS0, ctx->x, ctx->y,
+        k_flux_update<NS, 2><<<grid, kFluxThreads, 0, ctx->stream>>>(g, ctx->prim, ctx->CS, ctx->CS0, ctx->x, ctx->y,
                                                                     ctx->z, dt);
         break;
     case 3:
-        k_flux_tile<NS, 3><<<grid, kFluxThreads, 0, ctx->stream>>>(g, ctx->prim, ctx->CS, ctx->CS0, ctx->x, ctx->y,
+        k_flux_update<NS, 3><<<grid, kFluxThreads, 0, ctx->stream>>>(g, ctx->prim, ctx->CS, ctx->CS0, ctx->x, ctx->y,
                                                                     ctx->z, dt);
         break;
     default:
-        k_flux_tile<NS, 4><<<grid, kFluxThreads, 0, ctx->stream>>>(g, ctx->prim, ctx->CS, ctx->CS0, ctx->x, ctx->y,
+        k_flux_update<NS, 4><<<grid, kFluxThreads, 0, ctx->stream>>>(g, ctx->prim, ctx->CS, ctx->CS0, ctx->x, ctx->y,
                                                                     ctx->z, dt);
         break;
     }
