@@ -180,9 +180,19 @@ static void resolve_timing(eu3d_ctx *ctx)
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ bool interior_cell(const Grid &g, long long t, int &i, int &j, int &k)
 {
-    long long nk = g.nk, nj = g.nj;
-    if (t >= (long long)g.ni * nj * nk)
+    const long long n = (long long)g.ni * g.nj * g.nk;
+    if (t >= n)
         return false;
+    if (n <= 0xFFFFFFFFll) { // 32-bit division: a 64-bit one is a long emulated sequence
+        const unsigned u = (unsigned)t, nk = (unsigned)g.nk, nj = (unsigned)g.nj;
+        const unsigned r = u / nk;
+        k = (int)(u - r * nk) + g.bc;
+        const unsigned q = r / nj;
+        j = (int)(r - q * nj) + g.bc;
+        i = (int)q + g.bc;
+        return true;
+    }
+    long long nk = g.nk, nj = g.nj;
     k = (int)(t % nk) + g.bc;
     long long r = t / nk;
     j = (int)(r % nj) + g.bc;

@@ -29,86 +29,40 @@ using eu3d_mlp_internal::kOut;
 namespace {
 
 constexpr int kLayers = 4;
+// tiles of the hidden-layer GEMMs: a wave computes kWM x kWN 32 x 32 tiles
+#ifndef EU3D_MLP_WM
+#define EU3D_MLP_WM 2
+#endif
+#ifndef EU3D_MLP_WN
+#define EU3D_MLP_WN 2
+#endif
+constexpr int kWM = EU3D_MLP_WM, kWN = EU3D_MLP_WN;
 
 typedef short s16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 __device__ __forceinline__ float gelu(float x) { return 0.5f * x * (1.0f + erff(x * 0.70710678118654752f)); }
-__device__ __forceinline__ float gelu_grad(float x)
+// gelu(x) and gelu'(x) = 0.5 (1 + erf(x / sqrt 2)) + x phi(x) with one erf
+__device__ __forceinline__ void gelu_both(float x, float &h, float &gp)
 {
-    return 0.5f * (1.0f + erff(x * 0.70710678118654752f)) + x * 0.39894228040143268f * expf(-0.5f * x * x);
+    const float e = erff(x * 0.70710678118654752f);
+    h = 0.5f * x * (1.0f + e);
+    gp = 0.5f * (1.0f + e) + x * 0.39894228040143268f * expf(-0.5f * x * x);
 }
 
 enum Epi {
-    EPI_BIAS_GELU = 0, // Z = acc + b (fp32), H = gelu(Z) (bf16)
+    EPI_BIAS_GELU = 0, // Z = acc + b: H = gelu(Z) (bf16), G = gelu'(Z) (fp32, for the backward)
     EPI_BIAS = 1,      // Y = acc + b (fp32)
-    EPI_GELU_BWD = 2,  // dZ = acc * gelu'(Zaux) (bf16)
-    EPI_ATOMIC = 3,    // C += acc (fp32, split over the reduction)
+    EPI_GELU_BWD = 2,  // dZ = acc * Gaux (bf16), and colsum[n] += sum over the rows of dZ
     EPI_GELU_ONLY = 4, // H = gelu(acc + b) (bf16), inference: Z not kept
 };
 
-// C[M x N] = A[M x K] . B[K x N] with general strides, bf16 operands, fp32
-// accumulation on v_mfma_f32_32x32x16_bf16.  A wave computes a 32 x 32 tile;
-// 4 waves per block (64 x 64).  M, N multiples of 32; K (per split) multiple
-// of 16.  blockIdx.z splits the reduction (EPI_ATOMIC only).
-// Operand layout of the instruction: lane l holds A[l%32][8*(l/32)+t] and
-// B[8*(l/32)+t][l%32], t = 0..7; accumulator v of lane l is
-// C[8*(v/4) + 4*(l/32) + v%4][l%32].
-template <int EPI>
-__global__ void __launch_bounds__(256)
-    k_gemm(int M, int N, int K, const bf16 *__restrict__ A, long long sAm, long long sAk, const bf16 *__restrict__ B,
-           long long sBk, long long sBn, const float *__restrict__ bias, float *__restrict__ Cf,
-           bf16 *__restrict__ Cb, long long ldc, const float *__restrict__ Zaux)
-{
-    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
-    const int m0 = blockIdx.x * 64 + (w & 1) * 32;
-    const int n0 = blockIdx.y * 64 + (w >> 1) * 32;
-    if (m0 >= M || n0 >= N)
-        return;
-    const int kspl = K / gridDim.z;
-    const int kbeg = blockIdx.z * kspl, kend = kbeg + kspl;
-    const int r = l & 31, q = l >> 5;
-    f32x16 acc;
-#pragma unroll
-    for (int v = 0; v < 16; v++)
-        acc[v] = 0.f;
-    const bf16 *Ap = A + (long long)(m0 + r) * sAm;
-    const bf16 *Bp = B + (long long)(n0 + r) * sBn;
-    for (int k0 = kbeg; k0 < kend; k0 += 16) {
-        s16x8 a, b;
-#pragma unroll
-        for (int t = 0; t < 8; t++) {
-            long long k = k0 + 8 * q + t;
-            a[t] = (short)Ap[k * sAk];
-            b[t] = (short)Bp[k * sBk];
-        }
-        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
-    }
-#pragma unroll
-    for (int v = 0; v < 16; v++) {
-        const int m = m0 + 8 * (v >> 2) + 4 * q + (v & 3);
-        const int n = n0 + r;
-        const long long o = (long long)m * ldc + n;
-        float x = acc[v];
-        if (EPI == EPI_BIAS_GELU) {
-            float z = x + bias[n];
-            Cf[o] = z;
-            Cb[o] = f2bf(gelu(z));
-        } else if (EPI == EPI_BIAS) {
-            Cf[o] = x + bias[n];
-        } else if (EPI == EPI_GELU_ONLY) {
-            Cb[o] = f2bf(gelu(x + bias[n]));
-        } else if (EPI == EPI_GELU_BWD) {
-            Cb[o] = f2bf(x * gelu_grad(Zaux[o]));
-        } else {
-            atomicAdd(&Cf[o], x);
-        }
-    }
-}
-
-// C[M x N] = A[M x K] . Bt[N x K]^T with both operands contiguous in k
-// (row strides lda, ldb multiples of 8): every lane fetches its 8 k-values
-// of a 32-row fragment with one 16-byte load.  A wave computes WM x WN tiles
+// C[M x N] = A[M x K] . Bt[N x K]^T, bf16 operands, fp32 accumulation on
+// v_mfma_f32_32x32x16_bf16, both operands contiguous in k (row strides lda,
+// ldb multiples of 8): every lane fetches its 8 k-values of a 32-row
+// fragment with one 16-byte load.  Operand layout of the instruction: lane
+// l holds A[l%32][8*(l/32)+t] and B[8*(l/32)+t][l%32], t = 0..7;
+// accumulator v of lane l is C[8*(v/4) + 4*(l/32) + v%4][l%32].  A wave computes WM x WN tiles
 // of 32 x 32 (fragments reused across the tile), a block is 4 waves stacked
 // in m: (4 WM 32) x (WN 32) outputs.  M multiple of 32 WM, N of 32 WN, K of 16.
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
@@ -116,7 +70,7 @@ template <int EPI, int WM, int WN>
 __global__ void __launch_bounds__(256)
     k_gemm_nt(int M, int N, int K, const bf16 *__restrict__ A, int lda, const bf16 *__restrict__ Bt, int ldb,
               const float *__restrict__ bias, float *__restrict__ Cf, bf16 *__restrict__ Cb, int ldc,
-              const float *__restrict__ Zaux)
+              const float *__restrict__ Gaux, float *__restrict__ colsum)
 {
     const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
     const int m0 = (blockIdx.x * 4 + w) * (32 * WM);
@@ -157,6 +111,10 @@ __global__ void __launch_bounds__(256)
             for (int b = 0; b < WN; b++)
                 acc[a][b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(af[a], bfr[b], acc[a][b], 0, 0, 0);
     }
+    float cs[WN];
+#pragma unroll
+    for (int b = 0; b < WN; b++)
+        cs[b] = 0.f;
 #pragma unroll
     for (int a = 0; a < WM; a++)
 #pragma unroll
@@ -168,17 +126,92 @@ __global__ void __launch_bounds__(256)
                 const long long o = (long long)m * ldc + n;
                 float x = acc[a][b][v];
                 if (EPI == EPI_BIAS_GELU) {
-                    float z = x + bias[n];
-                    Cf[o] = z;
-                    Cb[o] = f2bf(gelu(z));
+                    float h, gp;
+                    gelu_both(x + bias[n], h, gp);
+                    Cf[o] = gp;
+                    Cb[o] = f2bf(h);
                 } else if (EPI == EPI_GELU_ONLY) {
                     Cb[o] = f2bf(gelu(x + bias[n]));
                 } else if (EPI == EPI_BIAS) {
                     Cf[o] = x + bias[n];
                 } else if (EPI == EPI_GELU_BWD) {
-                    Cb[o] = f2bf(x * gelu_grad(Zaux[o]));
+                    const bf16 d = f2bf(x * Gaux[o]);
+                    Cb[o] = d;
+                    cs[b] += bf2f(d);
                 }
             }
+    if (EPI == EPI_GELU_BWD) {
+        // lanes l and l + 32 hold the same column; one atomic per column
+#pragma unroll
+        for (int b = 0; b < WN; b++) {
+            const float c = cs[b] + __shfl_xor(cs[b], 32, 64);
+            if (q == 0)
+                atomicAdd(&colsum[n0 + 32 * b + r], c);
+        }
+    }
+}
+
+// Weight gradient C[M x N] += A^T . D for A [R x M] and D [R x N], both
+// row-major bf16 (the layer input and dZ over a batch of R rows): the
+// reduction runs over rows, which are strided in both operands.  A block
+// owns a 64 x 64 tile of C (4 waves of 32 x 32) and a 1/gridDim.z share of
+// the rows; per chunk of 32 rows it stages both operands in LDS transposed
+// (row index fastest), so that every lane reads its 8 reduction values of an
+// MFMA fragment with one 16-byte LDS load.  Partial tiles of a split are
+// added to C with FP32 atomics.  M, N multiples of 32, R of 32 per split.
+constexpr int kWgRows = 32;        // rows per LDS chunk
+constexpr int kWgLd = kWgRows + 8; // padded LDS row (bf16)
+__global__ void __launch_bounds__(256)
+    k_gemm_wgrad(int M, int N, int R, const bf16 *__restrict__ A, const bf16 *__restrict__ D, float *__restrict__ C)
+{
+    __shared__ __attribute__((aligned(16))) bf16 sA[64 * kWgLd];
+    __shared__ __attribute__((aligned(16))) bf16 sD[64 * kWgLd];
+    const int t = threadIdx.x, w = t >> 6, l = t & 63;
+    const int mt = blockIdx.x * 64, nt = blockIdx.y * 64;
+    const int wm = (w & 1) * 32, wn = (w >> 1) * 32;
+    const bool act = (mt + wm < M) && (nt + wn < N);
+    const int rows = R / gridDim.z, rbeg = blockIdx.z * rows, rend = rbeg + rows;
+    // staging: thread t moves 8 consecutive columns (one 16-byte load) of
+    // chunk row t / 8 for both operands
+    const int sr = t >> 3, sc = (t & 7) * 8;
+    const int r = l & 31, q = l >> 5;
+    f32x16 acc;
+#pragma unroll
+    for (int v = 0; v < 16; v++)
+        acc[v] = 0.f;
+    for (int r0 = rbeg; r0 < rend; r0 += kWgRows) {
+        const long long gr = r0 + sr;
+        u32x4 va = {0u, 0u, 0u, 0u}, vd = {0u, 0u, 0u, 0u};
+        if (mt + sc < M)
+            va = *reinterpret_cast<const u32x4 *>(A + gr * M + mt + sc);
+        if (nt + sc < N)
+            vd = *reinterpret_cast<const u32x4 *>(D + gr * N + nt + sc);
+        const s16x8 ea = __builtin_bit_cast(s16x8, va), ed = __builtin_bit_cast(s16x8, vd);
+#pragma unroll
+        for (int e = 0; e < 8; e++) {
+            sA[(sc + e) * kWgLd + sr] = (bf16)ea[e];
+            sD[(sc + e) * kWgLd + sr] = (bf16)ed[e];
+        }
+        __syncthreads();
+        if (act) {
+#pragma unroll
+            for (int kk = 0; kk < kWgRows; kk += 16) {
+                const u32x4 fa = *reinterpret_cast<const u32x4 *>(&sA[(wm + r) * kWgLd + kk + 8 * q]);
+                const u32x4 fd = *reinterpret_cast<const u32x4 *>(&sD[(wn + r) * kWgLd + kk + 8 * q]);
+                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(s16x8, fa),
+                                                              __builtin_bit_cast(s16x8, fd), acc, 0, 0, 0);
+            }
+        }
+        __syncthreads();
+    }
+    if (act) {
+#pragma unroll
+        for (int v = 0; v < 16; v++) {
+            const int m = mt + wm + 8 * (v >> 2) + 4 * q + (v & 3);
+            const int n = nt + wn + r;
+            atomicAdd(&C[(long long)m * N + n], acc[v]);
+        }
+    }
 }
 
 // features (raw, n_in per row, gathered through idx) -> normalized bf16 [B x kIn];
@@ -219,21 +252,35 @@ __global__ void k_loss(int B, int n_out, const float *__restrict__ Y, const floa
         atomicAdd(loss, part / ((double)B * n_out));
 }
 
-// db[n] += sum_b dZ[b][n]
-__global__ void k_colsum(int B, int N, const bf16 *__restrict__ dZ, float *__restrict__ db)
+// db[n] += sum_b dZ[b][n].  A block of 256 threads covers kColRows rows and
+// CW = min(N, 256) columns (blockIdx.y selects the column chunk): thread t
+// sums column t % CW of the rows t / CW, t / CW + step, ... (the block reads
+// whole row segments, coalesced), the partial sums of one column meet in
+// LDS, and one atomic per column and block goes to db.
+constexpr int kColRows = 128;
+__global__ void __launch_bounds__(256) k_colsum(int B, int N, const bf16 *__restrict__ dZ, float *__restrict__ db)
 {
-    int n = blockIdx.y * blockDim.x + threadIdx.x;
-    if (n >= N)
-        return;
-    int rows = (B + gridDim.x - 1) / gridDim.x;
-    int b0 = blockIdx.x * rows, b1 = min(B, b0 + rows);
+    __shared__ float part[256];
+    const int CW = min(N, 256), step = 256 / CW;
+    const int t = threadIdx.x, n = blockIdx.y * 256 + t % CW, r0 = t / CW;
+    const long long b0 = (long long)blockIdx.x * kColRows;
     float s = 0.f;
-    for (int b = b0; b < b1; b++)
-        s += bf2f(dZ[(long long)b * N + n]);
-    atomicAdd(&db[n], s);
+    if (r0 < step && n < N) {
+        for (int r = r0; r < kColRows; r += step) {
+            const long long b = b0 + r;
+            if (b < B)
+                s += bf2f(dZ[b * N + n]);
+        }
+    }
+    part[t] = s;
+    __syncthreads();
+    if (t < CW && n < N) {
+        for (int u = 1; u < step; u++)
+            s += part[t + u * CW];
+        atomicAdd(&db[n], s);
+    }
 }
 
-// Adam on fp32 master parameters; refreshes the bf16 copy
 // Adam on fp32 master parameters; refreshes the bf16 copy and, for a weight
 // matrix [rows x cols] (cols > 0), its transposed bf16 copy.
 __global__ void k_adam(long long n, float *__restrict__ p, float *__restrict__ g, float *__restrict__ m,
@@ -283,7 +330,7 @@ struct eu3d_mlp {
     bf16 *Wb[kLayers], *Wt[kLayers]; // bf16 weights [K x N] and transposed [N x K]
     float *norm; // in mean[kIn], in std[kIn], out mean[kOut], out std[kOut]
     bf16 *X0, *Hb[kLayers - 1], *dZ[kLayers];
-    float *Z[kLayers - 1], *Y, *T0;
+    float *G[kLayers - 1], *Y, *T0; // G: gelu'(pre-activation) of the hidden layers
     double *loss;
     float *stage;      // [max_batch x (n_in + n_out)] host-batch staging
     long long *perm;   // [ds_n] device copy of the epoch's shuffle
@@ -305,19 +352,19 @@ struct eu3d_mlp {
 static int mlp_forward(eu3d_mlp *mlp, hipStream_t st, int B, bool keep_z)
 {
     const int H = mlp->H;
-    const dim3 blk(256), grid_h((B + 255) / 256, H / 64), grid_o((B + 255) / 256, 1);
+    const dim3 blk(256), grid_h((B + 128 * kWM - 1) / (128 * kWM), H / (32 * kWN)), grid_o((B + 255) / 256, 1);
     for (int l = 0; l < kLayers - 1; l++) {
         const bf16 *in = l == 0 ? mlp->X0 : mlp->Hb[l - 1];
         const int K = mlp->K[l];
         if (keep_z)
-            k_gemm_nt<EPI_BIAS_GELU, 2, 2><<<grid_h, blk, 0, st>>>(B, H, K, in, K, mlp->Wt[l], K, mlp->b[l],
-                                                                   mlp->Z[l], mlp->Hb[l], H, nullptr);
+            k_gemm_nt<EPI_BIAS_GELU, kWM, kWN><<<grid_h, blk, 0, st>>>(B, H, K, in, K, mlp->Wt[l], K, mlp->b[l],
+                                                                   mlp->G[l], mlp->Hb[l], H, nullptr, nullptr);
         else
-            k_gemm_nt<EPI_GELU_ONLY, 2, 2><<<grid_h, blk, 0, st>>>(B, H, K, in, K, mlp->Wt[l], K, mlp->b[l], nullptr,
-                                                                   mlp->Hb[l], H, nullptr);
+            k_gemm_nt<EPI_GELU_ONLY, kWM, kWN><<<grid_h, blk, 0, st>>>(B, H, K, in, K, mlp->Wt[l], K, mlp->b[l], nullptr,
+                                                                   mlp->Hb[l], H, nullptr, nullptr);
     }
     k_gemm_nt<EPI_BIAS, 2, 1><<<grid_o, blk, 0, st>>>(B, kOut, H, mlp->Hb[kLayers - 2], H, mlp->Wt[kLayers - 1], H,
-                                                      mlp->b[kLayers - 1], mlp->Y, nullptr, kOut, nullptr);
+                                                      mlp->b[kLayers - 1], mlp->Y, nullptr, kOut, nullptr, nullptr);
     MCK(hipGetLastError());
     return EU3D_OK;
 }
@@ -328,16 +375,20 @@ static int mlp_backward(eu3d_mlp *mlp, int B)
     const bf16 *Ain[kLayers] = {mlp->X0, mlp->Hb[0], mlp->Hb[1], mlp->Hb[2]};
     for (int l = kLayers - 1; l >= 0; l--) {
         const int K = mlp->K[l], N = mlp->N[l];
-        // dW[K x N] += Ain^T . dZ_l, split over the batch
+        // dW[K x N] += Ain^T . dZ_l, the batch split over about 512 blocks
+        const int tiles = (int)(nblk(K, 64) * nblk(N, 64));
         int split = 1;
-        while (split < 64 && (B / (split * 2)) % 16 == 0 && B / (split * 2) >= 1024)
+        while (tiles * split * 2 <= 512 && (B / (split * 2)) % kWgRows == 0 && B % (split * 2) == 0)
             split *= 2;
-        k_gemm<EPI_ATOMIC><<<dim3(nblk(K, 64), nblk(N, 64), split), blk, 0, mlp->stream>>>(
-            K, N, B, Ain[l], 1, K, mlp->dZ[l], N, 1, nullptr, mlp->gW[l], nullptr, N, nullptr);
-        k_colsum<<<dim3(64, nblk(N, 64)), 64, 0, mlp->stream>>>(B, N, mlp->dZ[l], mlp->gb[l]);
-        if (l > 0) // dZ_{l-1} = (dZ_l . W_l^T) * gelu'(Z_{l-1}); W_l [K x N] is the k-contiguous operand
-            k_gemm_nt<EPI_GELU_BWD, 2, 2><<<dim3((B + 255) / 256, K / 64), blk, 0, mlp->stream>>>(
-                B, K, N, mlp->dZ[l], N, mlp->Wb[l], N, nullptr, nullptr, mlp->dZ[l - 1], K, mlp->Z[l - 1]);
+        k_gemm_wgrad<<<dim3(nblk(K, 64), nblk(N, 64), split), blk, 0, mlp->stream>>>(K, N, B, Ain[l], mlp->dZ[l],
+                                                                                    mlp->gW[l]);
+        if (l == kLayers - 1) // the other layers' bias gradients come with their dZ
+            k_colsum<<<dim3(nblk(B, kColRows), nblk(N, 256)), 256, 0, mlp->stream>>>(B, N, mlp->dZ[l], mlp->gb[l]);
+        if (l > 0) // dZ_{l-1} = (dZ_l . W_l^T) * gelu'(Z_{l-1}), db_{l-1}; W_l [K x N] is the k-contiguous operand
+            k_gemm_nt<EPI_GELU_BWD, kWM, kWN><<<dim3((B + 128 * kWM - 1) / (128 * kWM), K / (32 * kWN)), blk, 0,
+                                                mlp->stream>>>(
+                B, K, N, mlp->dZ[l], N, mlp->Wb[l], N, nullptr, nullptr, mlp->dZ[l - 1], K, mlp->G[l - 1],
+                mlp->gb[l - 1]);
     }
     MCK(hipGetLastError());
     return EU3D_OK;
@@ -363,8 +414,8 @@ int eu3d_mlp_destroy(eu3d_mlp *mlp)
     for (int l = 0; l < kLayers - 1; l++) {
         if (mlp->Hb[l])
             (void)hipFree(mlp->Hb[l]);
-        if (mlp->Z[l])
-            (void)hipFree(mlp->Z[l]);
+        if (mlp->G[l])
+            (void)hipFree(mlp->G[l]);
     }
     void *p[] = {mlp->norm, mlp->X0, mlp->Y, mlp->T0, mlp->loss, mlp->stage, mlp->perm, mlp->ds_x, mlp->ds_t};
     for (void *q : p)
@@ -435,7 +486,7 @@ int eu3d_mlp_create(int n_in, int hidden, int n_out, int max_batch, unsigned lon
     }
     for (int l = 0; l < kLayers - 1; l++) {
         MCK(hipMalloc(&mlp->Hb[l], (size_t)max_batch * H * 2));
-        MCK(hipMalloc(&mlp->Z[l], (size_t)max_batch * H * 4));
+        MCK(hipMalloc(&mlp->G[l], (size_t)max_batch * H * 4));
     }
     MCK(hipMalloc(&mlp->X0, (size_t)max_batch * kIn * 2));
     MCK(hipMalloc(&mlp->Y, (size_t)max_batch * kOut * 4));
