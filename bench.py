@@ -54,6 +54,38 @@ def surrogate_box():
     return r
 
 
+BF16_MFMA_PEAK_TFLOPS = 2500.0  # MI355X dense BF16 matrix
+
+
+def mlp_training_rate():
+    """Training throughput of the surrogate's network (MLP 12 -> 3 x 256 GELU
+    -> 10, BF16 MFMA, FP32 accumulation): epochs of 2^20 synthetic rows at
+    batch 65536 (forward, backward and Adam on the device), timed on the
+    host around the synchronous training call.  FLOPs per row: 3x the padded
+    forward 2 (32 H + 2 H^2 + 32 H), H = 256."""
+    import importlib
+    sur = importlib.import_module("eu3d-gpu_amd.surrogate")
+    H, n, B, epochs = 256, 1 << 20, 65536, 4
+    fwd = 2 * (32 * H + 2 * H * H + H * 32)
+    rng = np.random.default_rng(5)
+    x = rng.standard_normal((n, 12)).astype(np.float32)
+    t = rng.standard_normal((n, 10)).astype(np.float32)
+    net = sur.Network(12, 10, hidden=H, max_batch=B, seed=5)
+    net.fit_norm(x, t)
+    net.set_dataset(x, t)
+    net.train(1, B)
+    t0 = time.perf_counter()
+    net.train(epochs, B)
+    el = time.perf_counter() - t0
+    net.close()
+    tflops = 3 * fwd * n * epochs / el / 1e12
+    return {"bound": "mfma", "achieved": round(tflops, 2), "peak": BF16_MFMA_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "frac": round(tflops / BF16_MFMA_PEAK_TFLOPS, 4), "traffic": None,
+            "kernel": "eu3d_mlp_train epoch (k_gemm_nt forward/dZ, k_gemm_wgrad, k_adam, ...)",
+            "ms_per_epoch": round(el / epochs * 1e3, 3), "rows_per_epoch": n, "batch": B,
+            "flops_per_row": 3 * fwd}
+
+
 def eu3d_pkg():
     import importlib
     return importlib.import_module("eu3d-gpu_amd")
@@ -374,6 +406,7 @@ def main():
     s.close()
     if not args.no_surrogate:
         out["surrogate_speedup"] = surrogate_box()
+        out["roofline_mlp"] = mlp_training_rate()
     if not args.no_cpu_baseline:
         threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
         print(f"bench: cpu baseline with {threads} threads", file=sys.stderr, flush=True)

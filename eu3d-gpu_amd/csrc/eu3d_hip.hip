@@ -605,14 +605,6 @@ __global__ void __launch_bounds__(kFluxThreads) __attribute__((amdgpu_waves_per_
             if (own && l == 0)
                 zp0[q * kTJ + w] = Fzp[q];
         }
-        // the saved stage state of RK stages 2 and 3, loaded before the
-        // barrier wait
-        double s0[NV];
-        if (own && (MODE == 2 || MODE == 3)) {
-#pragma unroll
-            for (int q = 0; q < NV; q++)
-                s0[q] = CS0[q * N + c];
-        }
         __syncthreads(); // faces of plane i complete, tile of plane i no longer read
         if (own) {
             double cy = dt / (yn[jc] - yn[jc - 1]);
@@ -634,9 +626,9 @@ __global__ void __launch_bounds__(kFluxThreads) __attribute__((amdgpu_waves_per_
                     CS0[q * N + c] = u;
                     vv = u + rq;
                 } else if (MODE == 2) {
-                    vv = 0.75 * s0[q] + 0.25 * (u + rq);
+                    vv = 0.75 * CS0[q * N + c] + 0.25 * (u + rq);
                 } else {
-                    vv = (1.0 / 3.0) * s0[q] + (2.0 / 3.0) * (u + rq);
+                    vv = (1.0 / 3.0) * CS0[q * N + c] + (2.0 / 3.0) * (u + rq);
                 }
                 CS[q * N + c] = vv;
             }
