@@ -72,12 +72,12 @@ class Timing(ctypes.Structure):
 class SurrogateParams(ctypes.Structure):
     _fields_ = [("min_nchem", _i), ("tol_T", _d), ("tol_elem", _d), ("y_floor", _d),
                 ("hull_lo", ctypes.c_float * 32), ("hull_hi", ctypes.c_float * 32), ("proj", _d * 256),
-                ("rate_dt", _d), ("elem_abs", _d)]
+                ("rate_dt", _d), ("elem_abs", _d), ("T_min", _d), ("dT_max", _d)]
 
 
 class SurrogateStats(ctypes.Structure):
     _fields_ = [(n, ctypes.c_longlong) for n in ["routed", "fallback", "fail_hull", "fail_elem", "fail_neg",
-                                                 "fail_T"]]
+                                                 "fail_T", "fail_gate"]]
 
 
 _ll = ctypes.c_longlong

@@ -43,7 +43,7 @@ struct ChemCounters {
     unsigned int fb_count;
     unsigned int pad1;
     unsigned int pad2;
-    unsigned long long fail_hull, fail_elem, fail_neg, fail_T;
+    unsigned long long fail_hull, fail_elem, fail_neg, fail_T, fail_gate;
     // stiff-queue sort: bucket sizes and fill cursors (bucket = floor(log2 Nchem))
     unsigned int bucket_n[32];
     unsigned int bucket_cur[32];
@@ -58,6 +58,7 @@ struct SurDev {
     float lo[32], hi[32];
     double proj[16 * 16];
     double rate_dt, elem_abs;
+    double T_min, dT_max; // routing and small-change gates (0: off)
 };
 
 struct TimedKind {
@@ -81,6 +82,7 @@ struct eu3d_ctx {
     // learned surrogate (not owned) and its work lists
     eu3d_mlp *mlp;
     int sur_min;
+    double sur_tmin; // cells below this temperature stay with the FP64 integrator
     SurDev *dsur;
     int *sur_list;
     unsigned char *sur_ok;
@@ -743,6 +745,7 @@ __global__ void k_chem_reset(ChemCounters *c)
     c->fail_elem = 0;
     c->fail_neg = 0;
     c->fail_T = 0;
+    c->fail_gate = 0;
     for (int b = 0; b < 32; b++) {
         c->bucket_n[b] = 0;
         c->bucket_cur[b] = 0;
@@ -852,7 +855,7 @@ __device__ __forceinline__ unsigned long long lanemask_lt()
 template <int NS, bool FIRST, class MT>
 __device__ __forceinline__ int chem_step_cell(const Grid &g, const MT &m, long long c, double dtm, int n,
                                               double *CS, double *prim, double *T, double *E, double *C,
-                                              double *CMS, int keep_cms, int sur_min = 0)
+                                              double *CMS, int keep_cms, int sur_min = 0, double sur_tmin = 0.0)
 {
     const long long N = g.N;
     const double Tc = T[c];
@@ -876,7 +879,7 @@ __device__ __forceinline__ int chem_step_cell(const Grid &g, const MT &m, long l
         n = predict_nchem<NS>(R.W, Dcgs, Y, dtm);
         if (n < 1)
             return n;
-        if (sur_min > 0 && n >= sur_min) // left to the surrogate: no update here
+        if (sur_min > 0 && n >= sur_min && Tc >= sur_tmin) // left to the surrogate: no update here
             return -n;
     }
     double cms[NS], cs[NS + 4];
@@ -896,7 +899,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(EU3
     k_chem_first(Grid g, const DevMech *__restrict__ mp, double dtm, double *__restrict__ CS,
                  double *__restrict__ prim, double *__restrict__ T, double *__restrict__ E, double *__restrict__ C,
                  double *__restrict__ CMS, int *__restrict__ nchem, int *__restrict__ queue, ChemCounters *cnt,
-                 int compact, int keep_cms, int sur_min, int *__restrict__ sur_list)
+                 int compact, int keep_cms, int sur_min, int *__restrict__ sur_list, double sur_tmin = 0.0)
 {
     const auto &m = mech_ref<MT>(mp);
     long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
@@ -906,7 +909,8 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(EU3
     int n = 0;
     bool stiff = false, sur = false;
     if (active) {
-        n = chem_step_cell<NS, true>(g, m, c, dtm, 0, CS, prim, T, E, C, CMS, keep_cms, SUR ? sur_min : 0);
+        n = chem_step_cell<NS, true>(g, m, c, dtm, 0, CS, prim, T, E, C, CMS, keep_cms, SUR ? sur_min : 0,
+                                     sur_tmin);
         if (SUR && n < 0) {
             n = -n;
             sur = true;
@@ -1387,6 +1391,9 @@ __global__ void __launch_bounds__(kBlock)
         dY[s] = (double)(yr[s] * os[s] + om[s]) * scale;
     const double dT = (double)(yr[NS] * os[NS] + om[NS]) * scale;
     int fail = ok[row] ? 0 : 1;
+    // small-change gate: only cells the network moves little are taken
+    if (!fail && sp->dT_max > 0.0 && !(fabs(dT) <= sp->dT_max))
+        fail = 5;
     // element-conservation projection
     double dYp[NS], corr = 0.0, mag = 0.0;
 #pragma unroll
@@ -1428,10 +1435,11 @@ __global__ void __launch_bounds__(kBlock)
             fail = 4;
     }
     if (fail) {
-        unsigned long long *why = fail == 1 ? &cnt->fail_hull
+        unsigned long long *why = fail == 1   ? &cnt->fail_hull
                                   : fail == 2 ? &cnt->fail_elem
                                   : fail == 3 ? &cnt->fail_neg
-                                              : &cnt->fail_T;
+                                  : fail == 4 ? &cnt->fail_T
+                                              : &cnt->fail_gate;
         atomicAdd(why, 1ull);
         atomicAdd(&cnt->fb_count, 1u);
         queue[atomicAdd(&cnt->queue_count, 1u)] = ~(int)c;
@@ -1766,7 +1774,7 @@ static void launch_chem_sur(eu3d_ctx *ctx, double dt, int flags, int *rc)
     k_chem_reset<<<1, 1, 0, ctx->stream>>>(ctx->cnt);
     k_chem_first<NS, MT, true><<<blocks_for(ctx->n_int), kBlock, 0, ctx->stream>>>(
         g, ctx->mech, dt, ctx->CS, ctx->prim, ctx->T, ctx->E, ctx->C, ctx->CMS, ctx->nchem, ctx->queue, ctx->cnt, 1,
-        keep, ctx->sur_min, ctx->sur_list);
+        keep, ctx->sur_min, ctx->sur_list, ctx->sur_tmin);
     unsigned int nsur = 0;
     hipError_t e = hipMemcpyAsync(&nsur, &ctx->cnt->sur_count, sizeof nsur, hipMemcpyDeviceToHost, ctx->stream);
     if (e == hipSuccess)
@@ -2350,7 +2358,7 @@ int eu3d_set_surrogate(eu3d_ctx *ctx, eu3d_mlp *mlp, const eu3d_surrogate_params
     if (n_in != ctx->ns + 3 || n_out != ctx->ns + 1)
         FAIL(EU3D_ERR_ARG, "network shape must be ns+3 -> ns+1");
     if (p->min_nchem < 1 || !(p->tol_T > 0) || !(p->tol_elem > 0) || !(p->y_floor >= 0) || !(p->rate_dt >= 0) ||
-        !(p->elem_abs >= 0))
+        !(p->elem_abs >= 0) || !(p->T_min >= 0) || !(p->dT_max >= 0))
         FAIL(EU3D_ERR_ARG, "surrogate parameters out of range");
     SurDev h{};
     h.min_nchem = p->min_nchem;
@@ -2361,6 +2369,8 @@ int eu3d_set_surrogate(eu3d_ctx *ctx, eu3d_mlp *mlp, const eu3d_surrogate_params
     h.y_floor = p->y_floor;
     h.rate_dt = p->rate_dt;
     h.elem_abs = p->elem_abs;
+    h.T_min = p->T_min;
+    h.dT_max = p->dT_max;
     for (int q = 0; q < 32; q++) {
         h.lo[q] = p->hull_lo[q];
         h.hi[q] = p->hull_hi[q];
@@ -2381,6 +2391,7 @@ int eu3d_set_surrogate(eu3d_ctx *ctx, eu3d_mlp *mlp, const eu3d_surrogate_params
         ctx->sur_ok_cap = max_batch;
     }
     ctx->sur_min = p->min_nchem;
+    ctx->sur_tmin = p->T_min;
     ctx->mlp = mlp;
     return EU3D_OK;
 }
@@ -2398,6 +2409,7 @@ int eu3d_surrogate_stats_get(eu3d_ctx *ctx, eu3d_surrogate_stats *out)
     out->fail_elem = (long long)c.fail_elem;
     out->fail_neg = (long long)c.fail_neg;
     out->fail_T = (long long)c.fail_T;
+    out->fail_gate = (long long)c.fail_gate;
     return EU3D_OK;
 }
 

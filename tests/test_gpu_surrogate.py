@@ -228,6 +228,33 @@ def test_surrogate_full_fallback_is_fp64_bitwise(box, min_nchem):
     assert np.array_equal(a.nchem(), b.nchem())
 
 
+@pytest.mark.parametrize("gate", ["dT_max", "T_min"])
+def test_surrogate_gates_fall_back_to_fp64_bitwise(box, gate):
+    """dT_max: an update that moves T by more than dT_max is rejected (here
+    every one); T_min: colder cells are not routed (here none is).  Either way
+    the call gives the FP64 bits."""
+    a, b = box(), box()
+    T, p, Y = _states(a, 21)
+    for s in (a, b):
+        sur.set_cells(s.ctx, T, p, Y)
+    net, _, _ = _net(22)
+    kw = dict(dT_max=1e-300) if gate == "dT_max" else dict(T_min=1e5)
+    sur.attach(a.ctx, net, NAMES, min_nchem=2, **kw)
+    L = eu3d.load()
+    flags = eu3d.CHEM_COMPACT | eu3d.CHEM_KEEP_CMS
+    eu3d.check(L.eu3d_chemistry(a.ctx, DT, flags | eu3d.CHEM_SURROGATE), a.ctx, "chem sur")
+    eu3d.check(L.eu3d_chemistry(b.ctx, DT, flags), b.ctx, "chem")
+    st = sur.stats(a.ctx)
+    sa, sb = _copy_state(a), _copy_state(b)
+    for k in sa:
+        assert np.array_equal(sa[k], sb[k]), k
+    if gate == "dT_max":
+        assert st["routed"] > 0 and st["fallback"] == st["routed"]
+        assert st["fail_gate"] > 0 and st["fail_gate"] + st["fail_hull"] == st["routed"]
+    else:
+        assert st["routed"] == 0 and st["fallback"] == 0
+
+
 def test_surrogate_requires_network_and_compaction(box):
     s = box()
     L = eu3d.load()
