@@ -185,25 +185,46 @@ def odw_side_by_side(args):
         if mode == "surrogate":
             s.restart(dump, t_dev)
             net = sur.Network.load_file(args.model or sur.DEFAULT_MODEL, max_batch=65536)
-            sur.attach(s.ctx, net, names, min_nchem=args.min_nchem)
+            sur.attach(s.ctx, net, names, min_nchem=args.min_nchem, T_min=args.sur_tmin, dT_max=args.sur_dtmax,
+                       nchem_growth=args.sur_growth)
         f = flags | (eu3d.CHEM_SURROGATE if mode == "surrogate" else 0)
         eu3d.check(L.eu3d_synchronize(s.ctx), s.ctx)
         routed = fallback = 0
         failed = None
+        if mode == "fp64":  # HIP events around every kernel of the timed FP64 steps
+            eu3d.check(L.eu3d_timing_enable(s.ctx, 1), s.ctx)
+            tm = eu3d.Timing()
+            eu3d.check(L.eu3d_timing_get(s.ctx, tm), s.ctx)  # resets the accumulators
         t0 = time.perf_counter()
-        for step in range(args.steps):
+        if mode == "fp64":
+            try:
+                s.run(args.steps, f)
+            except eu3d.EU3DError as e:
+                failed = str(e)
+        for step in range(args.steps if mode == "surrogate" else 0):
             try:
                 s.run(1, f)
             except eu3d.EU3DError as e:
                 failed = f"step {step}: {e}"
+                try:
+                    _, hh = s.health()
+                    failed += (f" (invalid Nchem {hh.invalid_nchem}, non-finite {hh.nonfinite}, "
+                               f"max Nchem {hh.max_nchem})")
+                except eu3d.EU3DError:
+                    pass
                 print(f"bench: odw {mode} run stopped at {failed}", file=sys.stderr, flush=True)
                 break
-            if mode == "surrogate":
-                st = sur.stats(s.ctx)
-                routed += st["routed"]
-                fallback += st["fallback"]
+            st = sur.stats(s.ctx)
+            routed += st["routed"]
+            fallback += st["fallback"]
         eu3d.check(L.eu3d_synchronize(s.ctx), s.ctx)
         el = time.perf_counter() - t0
+        if mode == "fp64":
+            eu3d.check(L.eu3d_timing_get(s.ctx, tm), s.ctx)
+            eu3d.check(L.eu3d_timing_enable(s.ctx, 0), s.ctx)
+            kern = {"flux_update": tm.flux_total_ms, "recover": tm.recover_total_ms, "chemistry": tm.chem_total_ms,
+                    "ghost": tm.ghost_total_ms, "cfl": tm.cfl_total_ms}
+            flux_avg_ms = tm.flux_total_ms / max(tm.flux_calls, 1)
         if failed:
             res[mode] = dict(failed=failed, routed=routed, fallback=fallback)
             continue
@@ -213,6 +234,11 @@ def odw_side_by_side(args):
                          T=np.array(s.interior(s.field("T"))), ign=ignition_length(s), routed=routed,
                          fallback=fallback, stiff_last=h.stiff_cells, max_nchem_last=h.max_nchem)
     os.remove(dump)
+    flux_bytes = FLUX_BYTES_PER_NODE * n_int + (2 * RK_EXTRA_BYTES_PER_NODE * n_int) / 3.0
+    flux_gbs = flux_bytes / (flux_avg_ms * 1e-3) / 1e9
+    roof = {"bound": "hbm", "achieved": round(flux_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(flux_gbs / HBM_PEAK_GBS, 4), "traffic": None, "kernel": "k_flux_update",
+            "avg_ms": round(flux_avg_ms, 4), "bytes_per_launch": int(flux_bytes)}
     a, b = res["fp64"], res["surrogate"]
     if "failed" in a:
         raise RuntimeError(f"odw FP64 run failed: {a['failed']}")
@@ -223,6 +249,8 @@ def odw_side_by_side(args):
         "config": {"workload": f"oblique detonation, wedge frame {s.ni}x{s.nj}x{s.nk}, Mach 8 H2-air 1 atm 300 K, "
                                f"25 deg, timed from a developed front ({args.develop} steps)",
                    "grid": [s.ni, s.nj, s.nk]},
+        "roofline": roof,
+        "kernel_ms_per_step": {k: round(v / args.steps, 3) for k, v in kern.items()},
         "fp64": {"ms_per_step": a["ms_per_step"], "stiff_cells_last": a["stiff_last"],
                  "max_nchem_last": a["max_nchem_last"]},
         "surrogate": b if "failed" in b else {"min_nchem": args.min_nchem, "ms_per_step": b["ms_per_step"],
@@ -295,6 +323,12 @@ def main():
     ap.add_argument("--develop", type=int, default=3000, help="odw: steps before the timed region")
     ap.add_argument("--model", default=None, help="odw: surrogate model file (default: the committed one)")
     ap.add_argument("--min-nchem", type=int, default=2, help="odw: cells with Nchem >= this use the surrogate")
+    ap.add_argument("--sur-tmin", type=float, default=0.0, help="odw: route only cells at T >= this (K; 0: all)")
+    ap.add_argument("--sur-dtmax", type=float, default=0.0,
+                    help="odw: reject network updates with |dT| above this (K; 0: off)")
+    ap.add_argument("--sur-growth", type=float, default=0.0,
+                    help="odw: reject network updates whose new state predicts more than this times the old Nchem "
+                         "(0: off)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device once (torch) and exit: the known-size kernel that "
                          "scales the FETCH_SIZE/WRITE_SIZE counters")

@@ -72,7 +72,8 @@ class Timing(ctypes.Structure):
 class SurrogateParams(ctypes.Structure):
     _fields_ = [("min_nchem", _i), ("tol_T", _d), ("tol_elem", _d), ("y_floor", _d),
                 ("hull_lo", ctypes.c_float * 32), ("hull_hi", ctypes.c_float * 32), ("proj", _d * 256),
-                ("rate_dt", _d), ("elem_abs", _d), ("T_min", _d), ("dT_max", _d)]
+                ("rate_dt", _d), ("elem_abs", _d), ("T_min", _d), ("dT_max", _d),
+                ("nchem_growth", _d)]
 
 
 class SurrogateStats(ctypes.Structure):

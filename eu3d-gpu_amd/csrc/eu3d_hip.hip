@@ -59,6 +59,7 @@ struct SurDev {
     double proj[16 * 16];
     double rate_dt, elem_abs;
     double T_min, dT_max; // routing and small-change gates (0: off)
+    double nchem_growth;  // > 0: stiffness check of the new state (0: off)
 };
 
 struct TimedKind {
@@ -1367,7 +1368,7 @@ __global__ void __launch_bounds__(kBlock)
                 int count, double dt, const float *__restrict__ Y, const float *__restrict__ norm,
                 const unsigned char *__restrict__ ok, double *__restrict__ CS, double *__restrict__ prim,
                 double *__restrict__ T, double *__restrict__ E, double *__restrict__ C, double *__restrict__ CMS,
-                int keep_cms, int *__restrict__ queue, ChemCounters *cnt)
+                int keep_cms, int *__restrict__ queue, ChemCounters *cnt, const int *__restrict__ nchem)
 {
     const int row = blockIdx.x * blockDim.x + threadIdx.x;
     if (row >= count)
@@ -1391,7 +1392,8 @@ __global__ void __launch_bounds__(kBlock)
         dY[s] = (double)(yr[s] * os[s] + om[s]) * scale;
     const double dT = (double)(yr[NS] * os[NS] + om[NS]) * scale;
     int fail = ok[row] ? 0 : 1;
-    // small-change gate: only cells the network moves little are taken
+    // small-change gate: only cells the network moves little are taken (the
+    // gate failures, this one and the stiffness check below, count as fail_gate)
     if (!fail && sp->dT_max > 0.0 && !(fabs(dT) <= sp->dT_max))
         fail = 5;
     // element-conservation projection
@@ -1433,6 +1435,22 @@ __global__ void __launch_bounds__(kBlock)
         if (!(isfinite(st.T) && st.T > 200.0 && st.T < 6000.0 && st.P > 0.0 && st.G > 1.0 && isfinite(st.C) &&
               st.C > 0.0 && fabs(st.T - (T0 + dT)) <= sp->tol_T))
             fail = 4;
+    }
+    if (!fail && sp->nchem_growth > 0.0) {
+        // stiffness check: the sub-step count the next call would predict for
+        // the new state (same dt) must be valid and at most nchem_growth
+        // times the count of the state the network started from
+        double Mc[NS], Dcgs[NS];
+#pragma unroll
+        for (int s = 0; s < NS; s++) {
+            Mc[s] = cs[s] / m.mw[s] * 1000 * 1e-6;
+            Dcgs[s] = cs[s] * 1e-3;
+        }
+        Rates<NS> R;
+        rates<NS, false>(m, st.T, Mc, R);
+        const int n_new = predict_nchem<NS>(R.W, Dcgs, st.Y, dt);
+        if (n_new < 1 || (double)n_new > sp->nchem_growth * (double)nchem[c])
+            fail = 5;
     }
     if (fail) {
         unsigned long long *why = fail == 1   ? &cnt->fail_hull
@@ -1799,7 +1817,7 @@ static void launch_chem_sur(eu3d_ctx *ctx, double dt, int flags, int *rc)
         }
         k_sur_apply<NS, MT><<<blocks_for(count), kBlock, 0, ctx->stream>>>(
             g, ctx->mech, ctx->dsur, ctx->sur_list + off, count, dt, v.Y, v.norm, ctx->sur_ok, ctx->CS, ctx->prim,
-            ctx->T, ctx->E, ctx->C, ctx->CMS, keep, ctx->queue, ctx->cnt);
+            ctx->T, ctx->E, ctx->C, ctx->CMS, keep, ctx->queue, ctx->cnt, ctx->nchem);
     }
     sort_queue(ctx);
     k_chem_queue<NS, MT, true><<<ctx->ncu * 4, kBlock, 0, ctx->stream>>>(
@@ -2358,7 +2376,7 @@ int eu3d_set_surrogate(eu3d_ctx *ctx, eu3d_mlp *mlp, const eu3d_surrogate_params
     if (n_in != ctx->ns + 3 || n_out != ctx->ns + 1)
         FAIL(EU3D_ERR_ARG, "network shape must be ns+3 -> ns+1");
     if (p->min_nchem < 1 || !(p->tol_T > 0) || !(p->tol_elem > 0) || !(p->y_floor >= 0) || !(p->rate_dt >= 0) ||
-        !(p->elem_abs >= 0) || !(p->T_min >= 0) || !(p->dT_max >= 0))
+        !(p->elem_abs >= 0) || !(p->T_min >= 0) || !(p->dT_max >= 0) || !(p->nchem_growth >= 0))
         FAIL(EU3D_ERR_ARG, "surrogate parameters out of range");
     SurDev h{};
     h.min_nchem = p->min_nchem;
@@ -2371,6 +2389,7 @@ int eu3d_set_surrogate(eu3d_ctx *ctx, eu3d_mlp *mlp, const eu3d_surrogate_params
     h.elem_abs = p->elem_abs;
     h.T_min = p->T_min;
     h.dT_max = p->dT_max;
+    h.nchem_growth = p->nchem_growth;
     for (int q = 0; q < 32; q++) {
         h.lo[q] = p->hull_lo[q];
         h.hi[q] = p->hull_hi[q];

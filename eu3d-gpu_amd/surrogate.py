@@ -243,11 +243,13 @@ ACCEPT = dict(min_nchem=2, tol_T=25.0, tol_elem=0.1, elem_abs=5e-4, y_floor=1e-3
 
 
 def attach(ctx, net, names, min_nchem=ACCEPT["min_nchem"], tol_T=ACCEPT["tol_T"], tol_elem=ACCEPT["tol_elem"],
-           y_floor=ACCEPT["y_floor"], elem_abs=ACCEPT["elem_abs"], hull=None, T_min=0.0, dT_max=0.0):
+           y_floor=ACCEPT["y_floor"], elem_abs=ACCEPT["elem_abs"], hull=None, T_min=0.0, dT_max=0.0,
+           nchem_growth=0.0):
     """eu3d_set_surrogate with the element projector of the species list and
     the network's training box (or hull=(lo, hi)).  T_min > 0 routes only
     cells at or above that temperature; dT_max > 0 rejects updates whose
-    |dT| exceeds it (both 0: off)."""
+    |dT| exceeds it; nchem_growth > 0 rejects new states that the next call
+    would find much stiffer (all 0: off)."""
     L = load()
     A, _ = element_matrix(names)
     P = projector(A)
@@ -256,7 +258,7 @@ def attach(ctx, net, names, min_nchem=ACCEPT["min_nchem"], tol_T=ACCEPT["tol_T"]
     prm.min_nchem, prm.tol_T, prm.tol_elem, prm.y_floor = min_nchem, tol_T, tol_elem, y_floor
     prm.rate_dt = float(net.rate_dt)
     prm.elem_abs = float(elem_abs)
-    prm.T_min, prm.dT_max = float(T_min), float(dT_max)
+    prm.T_min, prm.dT_max, prm.nchem_growth = float(T_min), float(dT_max), float(nchem_growth)
     lo, hi = hull if hull is not None else (net.hull_lo, net.hull_hi)
     for q in range(32):
         prm.hull_lo[q] = float(lo[q]) if q < len(lo) else 0.0

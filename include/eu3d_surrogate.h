@@ -79,7 +79,9 @@ int eu3d_mlp_train(eu3d_mlp *mlp, int epochs, int B, float lr, float lr_end, uns
  *  - the state recovered from the new composition at the cell's internal
  *    energy (the Newton of Update_after_CS) has 200 K < T < 6000 K, p > 0,
  *    gamma > 1 and a finite sound speed, and T within tol_T of T + dT_net;
- *  - with dT_max > 0, |dT_net| <= dT_max.
+ *  - with dT_max > 0, |dT_net| <= dT_max;
+ *  - with nchem_growth > 0, the new state is not much stiffer (rates at the
+ *    new state: predicted Nchem valid and <= nchem_growth x the old one).
  * With T_min > 0 only cells at T >= T_min are routed at all. */
 typedef struct {
     int min_nchem;      /* cells with predicted Nchem >= min_nchem (>= 1) use the network */
@@ -93,6 +95,8 @@ typedef struct {
     double elem_abs;    /* absolute part of the element check (mass fraction) */
     double T_min;       /* > 0: cells colder than this (K) are not routed to the network */
     double dT_max;      /* > 0: an update whose |dT_net| exceeds this (K) is rejected (small-change gate) */
+    double nchem_growth; /* > 0: reject a new state whose predicted Nchem (TrapezoidPrediction at the same
+                            dt) is invalid or exceeds nchem_growth x the cell's Nchem before the update */
 } eu3d_surrogate_params;
 
 /* Attach (mlp != NULL) or detach (mlp == NULL) a network; the network must
@@ -107,7 +111,7 @@ typedef struct {
     long long routed;   /* cells sent to the network */
     long long fallback; /* of those, integrated by the FP64 trapezoid after a failed check */
     long long fail_hull, fail_elem, fail_neg, fail_T;
-    long long fail_gate; /* rejected by the small-change gate (dT_max) */
+    long long fail_gate; /* rejected by the small-change gate (dT_max) or the stiffness check */
 } eu3d_surrogate_stats;
 int eu3d_surrogate_stats_get(eu3d_ctx *ctx, eu3d_surrogate_stats *out);
 
