@@ -336,3 +336,18 @@ def test_committed_model_on_held_out_states():
     assert r["routed"] > 1000
     assert r["fallback"] <= 0.1 * r["routed"]
     assert r["T_abs_err_median"] < 5.0
+
+
+def test_committed_model_coupled_znd_front():
+    """The committed model coupled into the 1-D ZND initiation (2000 steps,
+    RK3 + MUSCL/AUSM+ + chemistry) with the routing (T >= 2500 K), small-change
+    (|dT| <= 5 K) and stiffness (predicted Nchem halves) gates: the detonation
+    front stays within one cell of the FP64 run and the peak temperature
+    within 1 %, with network updates actually accepted."""
+    net = sur.Network.load_file(sur.DEFAULT_MODEL, max_batch=65536)
+    r = sur.compare_znd(net, 2000, NAMES, T_min=2500.0, dT_max=5.0, nchem_growth=0.5)
+    print(r)
+    assert "error_surrogate" not in r, r
+    assert r["front_shift_cells"] <= 1
+    assert r["peakT_rel_err"] <= 0.01
+    assert r["routed"] - r["fallback"] > 10000
