@@ -160,3 +160,30 @@ def test_tecplot_series_output(tmp_path):
     assert np.allclose(blocks[9], y0, rtol=6e-6, atol=1e-300)
     assert (out / "output_1" / "load" / "Load_Loop.dat").exists()
     assert (out / "output_1" / "load" / "LoadDegree.dat").exists()
+
+
+def test_ctypes_structs_match_the_headers(tmp_path):
+    """Every ctypes mirror of a public struct has the C layout of its header:
+    a probe compiled with gcc prints sizeof and offsetof of each field."""
+    import ctypes
+    import subprocess
+    mirrors = {"eu3d_mesh": eu3d.Mesh, "eu3d_mechanism": eu3d.Mechanism, "eu3d_state": eu3d.State,
+               "eu3d_health_info": eu3d.Health, "eu3d_timing": eu3d.Timing,
+               "eu3d_surrogate_params": eu3d.SurrogateParams, "eu3d_surrogate_stats": eu3d.SurrogateStats}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "eu3d.h"', '#include "eu3d_surrogate.h"',
+             'int main(void) {']
+    for cname, cls in mirrors.items():
+        lines.append(f'printf("{cname} . %zu\\n", sizeof({cname}));')
+        for f in cls._fields_:
+            lines.append(f'printf("{cname} {f[0]} %zu\\n", offsetof({cname}, {f[0]}));')
+    lines += ['return 0;', '}']
+    src = tmp_path / "probe.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "probe"
+    subprocess.run(["gcc", "-std=c11", "-I", os.path.join(REPO, "include"), str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    got = {(a, b): int(c) for a, b, c in (ln.split() for ln in out if ln)}
+    for cname, cls in mirrors.items():
+        assert got[(cname, ".")] == ctypes.sizeof(cls), cname
+        for f in cls._fields_:
+            assert got[(cname, f[0])] == getattr(cls, f[0]).offset, (cname, f[0])
