@@ -385,40 +385,47 @@ __global__ void __launch_bounds__(kBlock) k_ghost_z(Grid g, int nprim, double *p
 // RHS = -dt/dx (F+ - F-) - dt/dy (G+ - G-) - dt/dz (Q+ - Q-) in that order
 // (TimeAdv::EE, src/TimeAdv.cpp:51-54).
 //
-// A block owns kTJ rows (j) of kTK = 64 cells (k, one wave per row) and
-// marches over kMarch planes (i); it has kTJ + 1 waves.  Per plane it stages
-// a tile of every primitive field -- rows j0-2 .. j0+kTJ+1, columns
-// k0-2 .. k0+kTK+1, ghost layers included -- in LDS once, and the y and z
+// A block has kTJ + 1 waves; a wave covers RW = 64 / KW rows (j) of KW cells
+// (k), so the block owns kTJ * RW rows and marches over kMarch planes (i).
+// KW = 64 on grids with >= 48 interior nodes in k, 32 or 16 on thinner ones,
+// so that short rows do not leave lanes idle.  Per plane the block stages a
+// tile of every primitive field -- rows j0-2 .. j0+rows+1, columns
+// k0-2 .. k0+KW+1, ghost layers included -- in LDS once, and the y and z
 // stencils read from there.  Every face flux is computed once:
 //   x faces: by the owning thread from HBM/L2; the +i face of plane i is
 //            kept in registers as the -i face of plane i+1;
-//   y faces: wave w < kTJ computes the face above its row (slot w+1), the
-//            extra wave the face below the tile (slot 0); exchanged in LDS;
-//   z faces: every owning thread computes the face at its +k side; the face
-//            at its -k side is the neighbour lane's (a wave shuffle), for
-//            lane 0 the one the extra wave computes (lanes < kTJ) into LDS.
+//   y faces: each owning thread computes the face above its row; the extra
+//            wave computes the faces below the tile; exchanged through LDS;
+//   z faces: each owning thread computes the face at its +k side; the face
+//            at its -k side is the neighbour lane's (a wave shuffle), for the
+//            first cell of a row the one the extra wave computes into LDS.
 // The next plane's tile is loaded after the update, so two barriers per
 // plane suffice; the own cell's conserved state is loaded before the face
 // work so that its latency overlaps it.
 // ---------------------------------------------------------------------------
-// Tile shape: 7 rows + 1 halo wave = 8 waves, which fill the 8 wave slots of
-// a CU at the kernel's register occupancy (2 waves per SIMD); the LDS tile
-// and faces take 146 KiB of the CU's 160 KiB; 8 planes per block give about
-// 5 rounds of blocks over the chip at 256 x 128 x 128.
+// Tile shape: 7 own waves + 1 halo wave = 8 waves, which fill the 8 wave
+// slots of a CU at the kernel's register occupancy (2 waves per SIMD); LDS
+// holds up to 146 KiB of the CU's 160 KiB; 8 planes per block give about 5
+// rounds of blocks over the chip at 256 x 128 x 128.
 #ifndef EU3D_FLUX_TJ
 #define EU3D_FLUX_TJ 7
 #endif
 #ifndef EU3D_FLUX_MARCH
 #define EU3D_FLUX_MARCH 8
 #endif
-constexpr int kTJ = EU3D_FLUX_TJ;
-constexpr int kTK = 64;
+constexpr int kTJ = EU3D_FLUX_TJ; // own waves per block
 constexpr int kFluxWaves = kTJ + 1;
 constexpr int kFluxThreads = 64 * kFluxWaves;
 constexpr int kMarch = EU3D_FLUX_MARCH;
-constexpr int kTR = kTJ + 4; // tile rows
-constexpr int kTC = kTK + 4; // tile columns
-constexpr int kXCD = 8;      // L2 domains (XCDs); workgroups are dealt to them round-robin
+constexpr int kXCD = 8; // L2 domains (XCDs); workgroups are dealt to them round-robin
+
+template <int KW>
+struct FluxShape {
+    static constexpr int kRW = 64 / KW;     // rows per wave
+    static constexpr int kRows = kTJ * kRW; // own rows per block
+    static constexpr int kTR = kRows + 4;   // tile rows
+    static constexpr int kTC = KW + 4;      // tile columns
+};
 
 // Tile of a flux block.  Consecutive workgroup ids land on different XCDs,
 // each with its own L2; the tiles are renumbered so that every XCD gets a
@@ -427,55 +434,63 @@ constexpr int kXCD = 8;      // L2 domains (XCDs); workgroups are dealt to them 
 struct FluxTile {
     int k0, j0, i0;
 };
+template <int KW>
 __device__ __forceinline__ bool flux_tile(const Grid &g, FluxTile &t)
 {
-    const unsigned nbk = (g.nk + kTK - 1) / kTK, nbj = (g.nj + kTJ - 1) / kTJ, nbi = (g.ni + kMarch - 1) / kMarch;
+    constexpr int R = FluxShape<KW>::kRows;
+    const unsigned nbk = (g.nk + KW - 1) / KW, nbj = (g.nj + R - 1) / R, nbi = (g.ni + kMarch - 1) / kMarch;
     const unsigned total = nbk * nbj * nbi, per = (total + kXCD - 1) / kXCD;
     const unsigned b = blockIdx.x;
     const unsigned tile = (b % kXCD) * per + b / kXCD;
     if (tile >= total)
         return false;
-    t.k0 = g.bc + (int)(tile % nbk) * kTK;
-    t.j0 = g.bc + (int)((tile / nbk) % nbj) * kTJ;
+    t.k0 = g.bc + (int)(tile % nbk) * KW;
+    t.j0 = g.bc + (int)((tile / nbk) % nbj) * R;
     t.i0 = g.bc + (int)(tile / (nbk * nbj)) * kMarch;
     return true;
 }
 
-static unsigned int flux_blocks(const Grid &g)
+static unsigned int flux_blocks(const Grid &g, int kw)
 {
-    const unsigned nbk = (g.nk + kTK - 1) / kTK, nbj = (g.nj + kTJ - 1) / kTJ, nbi = (g.ni + kMarch - 1) / kMarch;
+    const unsigned R = kTJ * (64 / kw);
+    const unsigned nbk = (g.nk + kw - 1) / kw, nbj = (g.nj + R - 1) / R, nbi = (g.ni + kMarch - 1) / kMarch;
     const unsigned total = nbk * nbj * nbi;
     return (total + kXCD - 1) / kXCD * kXCD;
 }
 
-// Stencil points from the LDS tile: base = row * kTC + column of the left
-// point, st = kTC (y) or 1 (z); fields kTR * kTC apart.
+// Stencil points from the LDS tile: base = row * TC + column of the left
+// point, st = TC (y) or 1 (z); fields FS = TR * TC apart.
+template <int FS>
 struct TileLd {
     const double *tile;
     int base, st;
-    __device__ __forceinline__ double operator()(int f, int o) const { return tile[f * (kTR * kTC) + base + o * st]; }
+    __device__ __forceinline__ double operator()(int f, int o) const { return tile[f * FS + base + o * st]; }
 };
 
-// Tile loads of one thread: (field, row) pairs are dealt to the waves, one
-// lane per column of the kTK interior columns (2 .. kTK+1); the 4 edge
-// columns of every pair (0, 1, kTK+2, kTK+3) are dealt to all threads.
-// Points outside the array (rows >= NJ or columns >= NK of a partial tile)
-// read as 0; no face that uses them is applied.  Rows and columns below the
-// tile are in the array: j0, k0 >= bc >= 2.
-template <int NP>
+// Tile loads of one thread: the KW interior columns (2 .. KW+1) of every
+// (field, row) pair are dealt to the threads, a row segment per KW lanes; the
+// 4 edge columns of every pair (0, 1, KW+2, KW+3) to all threads in order.  Points outside
+// the array (rows >= NJ or columns >= NK of a partial tile) read as 0; no
+// face that uses them is applied.  Rows and columns below the tile are in
+// the array: j0, k0 >= bc >= 2.
+template <int NP, int KW>
 struct TilePlan {
-    static constexpr int kPairs = NP * kTR;
-    static constexpr int kMain = (kPairs + kFluxWaves - 1) / kFluxWaves;
+    using S = FluxShape<KW>;
+    static constexpr int kPairs = NP * S::kTR;
+    static constexpr int kMain = (kPairs * KW + kFluxThreads - 1) / kFluxThreads;
     static constexpr int kEdges = kPairs * 4;
     static constexpr int kEdge = (kEdges + kFluxThreads - 1) / kFluxThreads;
     static constexpr int kRegs = kMain + kEdge;
 };
 
-template <int NP>
+// Main element m of thread (w, l) is column l % KW of pair
+// w RW + l / KW + m (512 / KW): for KW = 64 the pair is wave-uniform.
+template <int NP, int KW>
 __device__ __forceinline__ void tile_fetch(const Grid &g, const double *__restrict__ prim, int i, int j0, int k0,
-                                           int w, int l, int t, double (&v)[TilePlan<NP>::kRegs])
+                                           int w, int l, int t, double (&v)[TilePlan<NP, KW>::kRegs])
 {
-    using P = TilePlan<NP>;
+    using P = TilePlan<NP, KW>;
+    constexpr int TR = FluxShape<KW>::kTR, RW = FluxShape<KW>::kRW;
     const long long N = g.N;
     // formed per plane: strength-reduced across the march, the addresses
     // would be held in registers for the whole loop
@@ -486,9 +501,10 @@ __device__ __forceinline__ void tile_fetch(const Grid &g, const double *__restri
     // point outside the array is replaced by 0 afterwards
 #pragma unroll
     for (int m = 0; m < P::kMain; m++) {
-        const int p = min(w + kFluxWaves * m, P::kPairs - 1); // wave-uniform
-        const int f = p / kTR, tr = p - f * kTR;
-        const int j = j0 - 2 + tr, k = k0 + lane;
+        const int p = min(w * RW + lane / KW + (kFluxThreads / KW) * m, P::kPairs - 1);
+        const int col = lane % KW;
+        const int f = p / TR, tr = p - f * TR;
+        const int j = j0 - 2 + tr, k = k0 + col;
         const bool in = j < g.NJ && k < g.NK;
         const double x = prim[(long long)f * N + (plane + min(j, g.NJ - 1)) * g.NK + min(k, g.NK - 1)];
         v[m] = in ? x : 0.0;
@@ -497,47 +513,51 @@ __device__ __forceinline__ void tile_fetch(const Grid &g, const double *__restri
     for (int m = 0; m < P::kEdge; m++) {
         const int e = min(thr + kFluxThreads * m, P::kEdges - 1);
         const int p = e >> 2, c = e & 3;
-        const int f = p / kTR, tr = p - f * kTR;
-        const int j = j0 - 2 + tr, k = k0 - 2 + (c < 2 ? c : kTK + c);
+        const int f = p / TR, tr = p - f * TR;
+        const int j = j0 - 2 + tr, k = k0 - 2 + (c < 2 ? c : KW + c);
         const bool in = j < g.NJ && k < g.NK;
         const double x = prim[(long long)f * N + (plane + min(j, g.NJ - 1)) * g.NK + min(k, g.NK - 1)];
         v[P::kMain + m] = in ? x : 0.0;
     }
 }
 
-template <int NP>
-__device__ __forceinline__ void tile_store(double *tile, int w, int l, int t, const double (&v)[TilePlan<NP>::kRegs])
+template <int NP, int KW>
+__device__ __forceinline__ void tile_store(double *tile, int w, int l, int t,
+                                           const double (&v)[TilePlan<NP, KW>::kRegs])
 {
-    using P = TilePlan<NP>;
+    using P = TilePlan<NP, KW>;
+    constexpr int TC = FluxShape<KW>::kTC, RW = FluxShape<KW>::kRW;
 #pragma unroll
     for (int m = 0; m < P::kMain; m++) {
-        const int p = w + kFluxWaves * m;
+        const int p = w * RW + l / KW + (kFluxThreads / KW) * m;
         if (p < P::kPairs)
-            tile[p * kTC + 2 + l] = v[m];
+            tile[p * TC + 2 + l % KW] = v[m];
     }
 #pragma unroll
     for (int m = 0; m < P::kEdge; m++) {
         const int e = t + kFluxThreads * m;
         const int p = e >> 2, c = e & 3;
-        if (e < P::kEdges)
-            tile[p * kTC + (c < 2 ? c : kTK + c)] = v[P::kMain + m];
+        if (P::kEdges % kFluxThreads == 0 || e < P::kEdges)
+            tile[p * TC + (c < 2 ? c : KW + c)] = v[P::kMain + m];
     }
 }
 
-template <int NS, int MODE>
+template <int NS, int MODE, int KW>
 __global__ void __launch_bounds__(kFluxThreads) __attribute__((amdgpu_waves_per_eu(2)))
     k_flux_update(Grid g, const double *__restrict__ prim, double *__restrict__ CS, double *__restrict__ CS0,
                   const double *__restrict__ xn, const double *__restrict__ yn, const double *__restrict__ zn,
                   double dt)
 {
     constexpr int NV = NS + 4, NP = NS + 7;
-    using P = TilePlan<NP>;
-    __shared__ double tile[NP * kTR * kTC];
-    __shared__ double fy[NV * kFluxWaves * kTK]; // y faces, slot s between rows j0-1+s and j0+s
-    __shared__ double zb[NV * kTJ];              // z faces between k0-1 and k0, one per row
-    __shared__ double zp0[NV * kTJ];             // z faces between k0 and k0+1, one per row
+    using S = FluxShape<KW>;
+    using P = TilePlan<NP, KW>;
+    constexpr int ROWS = S::kRows, TC = S::kTC, FS = S::kTR * S::kTC;
+    __shared__ double tile[NP * FS];
+    __shared__ double fy[NV * (ROWS + 1) * KW]; // y faces, slot s between rows j0-1+s and j0+s
+    __shared__ double zb[NV * ROWS];            // z faces between k0-1 and k0, one per row
+    __shared__ double zp0[NV * ROWS];           // z faces between k0 and k0+1, one per row
     FluxTile tl;
-    if (!flux_tile(g, tl))
+    if (!flux_tile<KW>(g, tl))
         return;
     const int t = threadIdx.x, l = t & 63;
     const int w = __builtin_amdgcn_readfirstlane(t >> 6);
@@ -545,15 +565,18 @@ __global__ void __launch_bounds__(kFluxThreads) __attribute__((amdgpu_waves_per_
     const int k0 = tl.k0, j0 = tl.j0, i0 = tl.i0;
     const int i1 = min(i0 + kMarch, g.ni + g.bc);
     const int kend = g.nk + g.bc, jend = g.nj + g.bc;
-    const int jc = j0 + w, kc = k0 + l;
+    const int r = w * S::kRW + l / KW, kk = l % KW; // own row in the block, column
+    const int jc = j0 + r, kc = k0 + kk;
     const bool own = (w < kTJ) && (jc < jend) && (kc < kend);
-    const int ys = (w < kTJ) ? w + 1 : 0; // y face slot of this thread
-    const bool yact = (j0 - 1 + ys < jend) && (kc < kend);
-    const bool zhalo = (w == kTJ) && (l < kTJ) && (j0 + l < jend);
+    // the y face of this thread: above its row, or (extra wave) below the tile
+    const bool halo = w == kTJ;
+    const int ys = halo ? 0 : r + 1, ycol = halo ? l : kk;
+    const bool yact = halo ? (l < KW && k0 + l < kend) : own;
+    const bool zhalo = halo && (l < ROWS) && (j0 + l < jend);
 
     double v[P::kRegs];
-    tile_fetch<NP>(g, prim, i0, j0, k0, w, l, t, v);
-    tile_store<NP>(tile, w, l, t, v);
+    tile_fetch<NP, KW>(g, prim, i0, j0, k0, w, l, t, v);
+    tile_store<NP, KW>(tile, w, l, t, v);
     double rhs[NV], Fxm[NV];
     if (own)
         face_flux<NS>(0, prim, N, gidx(g, i0 - 1, jc, kc), g.sI, Fxm);
@@ -579,34 +602,34 @@ __global__ void __launch_bounds__(kFluxThreads) __attribute__((amdgpu_waves_per_
         }
         __syncthreads(); // tile of plane i complete
         // LDS addresses are loop-invariant; formed here, not held across the loop
-        int ybase = (ys + 1) * kTC + l + 2, zbase = (w + 2) * kTC + l + 2, hbase = (l + 2) * kTC + 1;
+        int ybase = (ys + 1) * TC + ycol + 2, zbase = (r + 2) * TC + kk + 2, hbase = (l + 2) * TC + 1;
         asm volatile("" : "+v"(ybase), "+v"(zbase), "+v"(hbase));
         if (yact) {
             double F[NV];
-            face_flux_ld<NS>(1, TileLd{tile, ybase, kTC}, F);
+            face_flux_ld<NS>(1, TileLd<FS>{tile, ybase, TC}, F);
 #pragma unroll
             for (int q = 0; q < NV; q++)
-                fy[(q * kFluxWaves + ys) * kTK + l] = F[q];
+                fy[(q * (ROWS + 1) + ys) * KW + ycol] = F[q];
         }
         double Fzp[NV] = {};
         if (own)
-            face_flux_ld<NS>(2, TileLd{tile, zbase, 1}, Fzp);
+            face_flux_ld<NS>(2, TileLd<FS>{tile, zbase, 1}, Fzp);
         if (zhalo) {
             double F[NV];
-            face_flux_ld<NS>(2, TileLd{tile, hbase, 1}, F);
+            face_flux_ld<NS>(2, TileLd<FS>{tile, hbase, 1}, F);
 #pragma unroll
             for (int q = 0; q < NV; q++)
-                zb[q * kTJ + l] = F[q];
+                zb[q * ROWS + l] = F[q];
         }
         // z differences Q+ - Q- before the barrier: Q- is the neighbour
-        // lane's Q+; lane 0 parks its Q+ in LDS and forms it after the
-        // barrier from the extra wave's face
+        // lane's Q+; the first cell of a row parks its Q+ in LDS and forms
+        // the difference after the barrier from the extra wave's face
         double dz[NV];
 #pragma unroll
         for (int q = 0; q < NV; q++) {
             dz[q] = Fzp[q] - __shfl_up(Fzp[q], 1);
-            if (own && l == 0)
-                zp0[q * kTJ + w] = Fzp[q];
+            if (own && kk == 0)
+                zp0[q * ROWS + r] = Fzp[q];
         }
         __syncthreads(); // faces of plane i complete, tile of plane i no longer read
         if (own) {
@@ -614,8 +637,8 @@ __global__ void __launch_bounds__(kFluxThreads) __attribute__((amdgpu_waves_per_
             double cz = dt / (zn[kc] - zn[kc - 1]);
 #pragma unroll
             for (int q = 0; q < NV; q++) {
-                double r1 = rhs[q] - cy * (fy[(q * kFluxWaves + w + 1) * kTK + l] - fy[(q * kFluxWaves + w) * kTK + l]);
-                double d = (l == 0) ? zp0[q * kTJ + w] - zb[q * kTJ + w] : dz[q];
+                double r1 = rhs[q] - cy * (fy[(q * (ROWS + 1) + r + 1) * KW + kk] - fy[(q * (ROWS + 1) + r) * KW + kk]);
+                double d = (kk == 0) ? zp0[q * ROWS + r] - zb[q * ROWS + r] : dz[q];
                 double rq = r1 - cz * d;
                 if (MODE == 4) { // IMEX: the right-hand side alone, into CS0
                     CS0[q * N + c] = rq;
@@ -637,8 +660,8 @@ __global__ void __launch_bounds__(kFluxThreads) __attribute__((amdgpu_waves_per_
             }
         }
         if (i + 1 < i1) {
-            tile_fetch<NP>(g, prim, i + 1, j0, k0, w, l, t, v);
-            tile_store<NP>(tile, w, l, t, v);
+            tile_fetch<NP, KW>(g, prim, i + 1, j0, k0, w, l, t, v);
+            tile_store<NP, KW>(tile, w, l, t, v);
         }
     }
 }
@@ -1701,31 +1724,50 @@ __global__ void k_derive_cp(Grid g, const DevMech *__restrict__ mp, const double
         FAIL(EU3D_ERR_ARG, "species count not compiled in (supported: 9)");                       \
     }
 
+template <int NS, int KW>
+static void launch_flux_kw(eu3d_ctx *ctx, int mode, double dt)
+{
+    const Grid &g = ctx->g;
+    const dim3 grid(flux_blocks(g, KW));
+    switch (mode) {
+    case 0:
+        k_flux_update<NS, 0, KW><<<grid, kFluxThreads, 0, ctx->stream>>>(g, ctx->prim, ctx->CS, ctx->CS0, ctx->x,
+                                                                        ctx->y, ctx->z, dt);
+        break;
+    case 1:
+        k_flux_update<NS, 1, KW><<<grid, kFluxThreads, 0, ctx->stream>>>(g, ctx->prim, ctx->CS, ctx->CS0, ctx->x,
+                                                                        ctx->y, ctx->z, dt);
+        break;
+    case 2:
+        k_flux_update<NS, 2, KW><<<grid, kFluxThreads, 0, ctx->stream>>>(g, ctx->prim, ctx->CS, ctx->CS0, ctx->x,
+                                                                        ctx->y, ctx->z, dt);
+        break;
+    case 3:
+        k_flux_update<NS, 3, KW><<<grid, kFluxThreads, 0, ctx->stream>>>(g, ctx->prim, ctx->CS, ctx->CS0, ctx->x,
+                                                                        ctx->y, ctx->z, dt);
+        break;
+    default:
+        k_flux_update<NS, 4, KW><<<grid, kFluxThreads, 0, ctx->stream>>>(g, ctx->prim, ctx->CS, ctx->CS0, ctx->x,
+                                                                        ctx->y, ctx->z, dt);
+        break;
+    }
+}
+
+// Row width of the flux tiles: 64 cells, or 32 / 16 on grids thinner in k.
+static int flux_row_width(const Grid &g) { return g.nk >= 48 ? 64 : g.nk >= 24 ? 32 : 16; }
+
 template <int NS, class MT>
 static void launch_flux(eu3d_ctx *ctx, int mode, double dt)
 {
-    const Grid &g = ctx->g;
-    const dim3 grid(flux_blocks(g));
-    switch (mode) {
-    case 0:
-        k_flux_update<NS, 0><<<grid, kFluxThreads, 0, ctx->stream>>>(g, ctx->prim, ctx->CS, ctx->CS0, ctx->x, ctx->y,
-                                                                    ctx->z, dt);
+    switch (flux_row_width(ctx->g)) {
+    case 64:
+        launch_flux_kw<NS, 64>(ctx, mode, dt);
         break;
-    case 1:
-        k_flux_update<NS, 1><<<grid, kFluxThreads, 0, ctx->stream>>>(g, ctx->prim, ctx->CS, ctx->CS0, ctx->x, ctx->y,
-                                                                    ctx->z, dt);
-        break;
-    case 2:
-        k_flux_update<NS, 2><<<grid, kFluxThreads, 0, ctx->stream>>>(g, ctx->prim, ctx->CS, ctx->CS0, ctx->x, ctx->y,
-                                                                    ctx->z, dt);
-        break;
-    case 3:
-        k_flux_update<NS, 3><<<grid, kFluxThreads, 0, ctx->stream>>>(g, ctx->prim, ctx->CS, ctx->CS0, ctx->x, ctx->y,
-                                                                    ctx->z, dt);
+    case 32:
+        launch_flux_kw<NS, 32>(ctx, mode, dt);
         break;
     default:
-        k_flux_update<NS, 4><<<grid, kFluxThreads, 0, ctx->stream>>>(g, ctx->prim, ctx->CS, ctx->CS0, ctx->x, ctx->y,
-                                                                    ctx->z, dt);
+        launch_flux_kw<NS, 16>(ctx, mode, dt);
         break;
     }
 }

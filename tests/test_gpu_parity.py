@@ -239,6 +239,23 @@ def test_sbi_rk3_cfl(tmp_path):
     compare_state(o, s, 1e-8, "SBI RK3 20 steps")
 
 
+@pytest.mark.parametrize("nk,nj", [(70, 10), (40, 18), (20, 30)])
+def test_flux_row_widths(tmp_path, nk, nj):
+    """The flux tiles of 64-, 32- and 16-cell rows (chosen by nk), each with a
+    ragged last tile in k and j: shock-bubble RK3 against the oracle."""
+    note(f"flux row widths nk {nk} nj {nj}")
+    ctrl = cases.sbi(str(tmp_path), ni=12, nj=nj, nk=nk)
+    o, s = pair(ctrl)
+    for step in range(3):
+        o.step(o.cfl(0.5, 1.0))
+        s.run(1)
+        if step == 0:
+            s.sync()
+            compare_state(o, s, 1e-12, f"SBI RK3 1 step nk {nk}")
+    s.sync()
+    compare_state(o, s, 1e-10, f"SBI RK3 3 steps nk {nk}")
+
+
 def test_znd_rk3(tmp_path):
     note("znd rk3 30 steps")
     ctrl = cases.znd(str(tmp_path), ni=201)
