@@ -909,7 +909,11 @@ __device__ __forceinline__ int chem_step_cell(const Grid &g, const MT &m, long l
     double cms[NS], cs[NS + 4];
     CellState<NS> st;
     chem_substep<NS>(m, x, R, Mc, dtm / n, cms, cs, st);
-    chem_store<NS>(N, cl, cs, st, cms, keep_cms, CS, CMS, prim, T, E, C);
+    // the store addresses are formed here: shared with the loads above, the
+    // ~40 field addresses would be held in registers through the sub-step
+    long long cw = cl;
+    asm volatile("" : "+v"(cw) : "v"(st.T));
+    chem_store<NS>(N, cw, cs, st, cms, keep_cms, CS, CMS, prim, T, E, C);
     return n;
 }
 
