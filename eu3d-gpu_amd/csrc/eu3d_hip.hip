@@ -704,7 +704,9 @@ __global__ void __launch_bounds__(kBlock)
         cs[q] = CS[q * N + c];
     CellState<NS> st;
     recover<NS>(mech_ref<MT>(mp), cs, T[c], E[c], st);
-    store_state<NS>(N, c, st, prim, T, E, C);
+    long long cw = c; // store addresses formed here, not held through the Newton
+    asm volatile("" : "+v"(cw) : "v"(st.T));
+    store_state<NS>(N, cw, st, prim, T, E, C);
 }
 
 // ---------------------------------------------------------------------------
