@@ -635,13 +635,15 @@ __global__ void __launch_bounds__(kFluxThreads) __attribute__((amdgpu_waves_per_
         if (own) {
             double cy = dt / (yn[jc] - yn[jc - 1]);
             double cz = dt / (zn[kc] - zn[kc - 1]);
+            long long cw = c; // addressed here, not held from the loads through the face work
+            asm volatile("" : "+v"(cw));
 #pragma unroll
             for (int q = 0; q < NV; q++) {
                 double r1 = rhs[q] - cy * (fy[(q * (ROWS + 1) + r + 1) * KW + kk] - fy[(q * (ROWS + 1) + r) * KW + kk]);
                 double d = (kk == 0) ? zp0[q * ROWS + r] - zb[q * ROWS + r] : dz[q];
                 double rq = r1 - cz * d;
                 if (MODE == 4) { // IMEX: the right-hand side alone, into CS0
-                    CS0[q * N + c] = rq;
+                    CS0[q * N + cw] = rq;
                     continue;
                 }
                 double u = u0[q];
@@ -649,14 +651,14 @@ __global__ void __launch_bounds__(kFluxThreads) __attribute__((amdgpu_waves_per_
                 if (MODE == 0) {
                     vv = u + rq;
                 } else if (MODE == 1) {
-                    CS0[q * N + c] = u;
+                    CS0[q * N + cw] = u;
                     vv = u + rq;
                 } else if (MODE == 2) {
-                    vv = 0.75 * CS0[q * N + c] + 0.25 * (u + rq);
+                    vv = 0.75 * CS0[q * N + cw] + 0.25 * (u + rq);
                 } else {
-                    vv = (1.0 / 3.0) * CS0[q * N + c] + (2.0 / 3.0) * (u + rq);
+                    vv = (1.0 / 3.0) * CS0[q * N + cw] + (2.0 / 3.0) * (u + rq);
                 }
-                CS[q * N + c] = vv;
+                CS[q * N + cw] = vv;
             }
         }
         if (i + 1 < i1) {
