@@ -285,20 +285,38 @@ __device__ __forceinline__ double muscl_upwind(bool up, double fm, double f0, do
 // AUSM+ interface flux (src/Flowfield.cpp:1797-1931).  dir 0/1/2 = x/y/z.
 // Field order: P D U V W H Gamma Y0..Y(NS-1).  ld(f, o) returns field f at
 // the point o (-1 .. 2) along the face direction, 0 being the left point.
+// ORDER 2: MUSCL_2 states; ORDER 1: MUSCL_1 (include/Differ.hpp:110-137),
+// the left state is the left point's value and the right state the right's.
 // ---------------------------------------------------------------------------
-template <int NS, class Ld>
+template <int ORDER>
+__device__ __forceinline__ double rec_left(double fm, double f0, double fp)
+{
+    return ORDER == 1 ? f0 : muscl_left(fm, f0, fp);
+}
+template <int ORDER>
+__device__ __forceinline__ double rec_right(double f0, double fp, double fpp)
+{
+    return ORDER == 1 ? fp : muscl_right(f0, fp, fpp);
+}
+template <int ORDER>
+__device__ __forceinline__ double rec_upwind(bool up, double fm, double f0, double fp, double fpp)
+{
+    return ORDER == 1 ? (up ? f0 : fp) : muscl_upwind(up, fm, f0, fp, fpp);
+}
+
+template <int NS, int ORDER = 2, class Ld>
 __device__ __forceinline__ void face_flux_ld(int dir, const Ld &ld, double (&F)[NS + 4])
 {
 #define LD(f, o) ld(f, o)
-    double PL = muscl_left(LD(0, -1), LD(0, 0), LD(0, 1));
-    double PR = muscl_right(LD(0, 0), LD(0, 1), LD(0, 2));
-    double DL = muscl_left(LD(1, -1), LD(1, 0), LD(1, 1));
-    double DR = muscl_right(LD(1, 0), LD(1, 1), LD(1, 2));
-    double GL = muscl_left(LD(6, -1), LD(6, 0), LD(6, 1));
-    double GR = muscl_right(LD(6, 0), LD(6, 1), LD(6, 2));
+    double PL = rec_left<ORDER>(LD(0, -1), LD(0, 0), LD(0, 1));
+    double PR = rec_right<ORDER>(LD(0, 0), LD(0, 1), LD(0, 2));
+    double DL = rec_left<ORDER>(LD(1, -1), LD(1, 0), LD(1, 1));
+    double DR = rec_right<ORDER>(LD(1, 0), LD(1, 1), LD(1, 2));
+    double GL = rec_left<ORDER>(LD(6, -1), LD(6, 0), LD(6, 1));
+    double GR = rec_right<ORDER>(LD(6, 0), LD(6, 1), LD(6, 2));
     const int fn = 2 + dir; // normal velocity field
-    double UnL = muscl_left(LD(fn, -1), LD(fn, 0), LD(fn, 1));
-    double UnR = muscl_right(LD(fn, 0), LD(fn, 1), LD(fn, 2));
+    double UnL = rec_left<ORDER>(LD(fn, -1), LD(fn, 0), LD(fn, 1));
+    double UnR = rec_right<ORDER>(LD(fn, 0), LD(fn, 1), LD(fn, 2));
 
     double CL = sqrt(GL * PL / DL);
     double CR = sqrt(GR * PR / DR);
@@ -334,12 +352,12 @@ __device__ __forceinline__ void face_flux_ld(int dir, const Ld &ld, double (&F)[
         if (d == dir)
             vel[d] = Un;
         else
-            vel[d] = muscl_upwind(up, LD(2 + d, -1), LD(2 + d, 0), LD(2 + d, 1), LD(2 + d, 2));
+            vel[d] = rec_upwind<ORDER>(up, LD(2 + d, -1), LD(2 + d, 0), LD(2 + d, 1), LD(2 + d, 2));
     }
-    double Hup = muscl_upwind(up, LD(5, -1), LD(5, 0), LD(5, 1), LD(5, 2));
+    double Hup = rec_upwind<ORDER>(up, LD(5, -1), LD(5, 0), LD(5, 1), LD(5, 2));
 #pragma unroll
     for (int s = 0; s < NS; s++) {
-        double Y = muscl_upwind(up, LD(7 + s, -1), LD(7 + s, 0), LD(7 + s, 1), LD(7 + s, 2));
+        double Y = rec_upwind<ORDER>(up, LD(7 + s, -1), LD(7 + s, 0), LD(7 + s, 1), LD(7 + s, 2));
         F[s] = VI * Dup * Y;
     }
     F[NS] = VI * Dup * vel[0];
@@ -358,11 +376,11 @@ struct GlobalLd {
     __device__ __forceinline__ double operator()(int f, int o) const { return prim[(long long)f * N + a + o * st]; }
 };
 
-template <int NS>
+template <int NS, int ORDER = 2>
 __device__ __forceinline__ void face_flux(int dir, const double *__restrict__ prim, long long N, long long a,
                                           long long st, double (&F)[NS + 4])
 {
-    face_flux_ld<NS>(dir, GlobalLd{prim, N, a, st}, F);
+    face_flux_ld<NS, ORDER>(dir, GlobalLd{prim, N, a, st}, F);
 }
 
 // ---------------------------------------------------------------------------

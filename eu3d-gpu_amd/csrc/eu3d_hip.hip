@@ -97,6 +97,7 @@ struct eu3d_ctx {
     double *inflow; // nprim + 2: P D U V W H Gamma Y.. T E
     bool has_state, cms_valid;
     bool generated; // tables bit-identical to MechH2JA919: use the compile-time kernels
+    int difference = 2; // MUSCL order of the face states (1: MUSCL_1, 2: MUSCL_2)
     bool generated_available;
     std::string err;
     bool timing;
@@ -542,7 +543,7 @@ __device__ __forceinline__ void tile_store(double *tile, int w, int l, int t,
     }
 }
 
-template <int NS, int MODE, int KW>
+template <int NS, int MODE, int KW, int ORDER>
 __global__ void __launch_bounds__(kFluxThreads) __attribute__((amdgpu_waves_per_eu(2)))
     k_flux_update(Grid g, const double *__restrict__ prim, double *__restrict__ CS, double *__restrict__ CS0,
                   const double *__restrict__ xn, const double *__restrict__ yn, const double *__restrict__ zn,
@@ -579,14 +580,14 @@ __global__ void __launch_bounds__(kFluxThreads) __attribute__((amdgpu_waves_per_
     tile_store<NP, KW>(tile, w, l, t, v);
     double rhs[NV], Fxm[NV];
     if (own)
-        face_flux<NS>(0, prim, N, gidx(g, i0 - 1, jc, kc), g.sI, Fxm);
+        face_flux<NS, ORDER>(0, prim, N, gidx(g, i0 - 1, jc, kc), g.sI, Fxm);
     for (int i = i0; i < i1; i++) {
         const long long c = gidx(g, i, jc, kc);
         if (own) {
             double Fxp[NV];
             long long ax = c;
             asm volatile("" : "+v"(ax)); // addressed per plane, not strength-reduced
-            face_flux<NS>(0, prim, N, ax, g.sI, Fxp);
+            face_flux<NS, ORDER>(0, prim, N, ax, g.sI, Fxp);
             double cx = -dt / (xn[i] - xn[i - 1]);
 #pragma unroll
             for (int q = 0; q < NV; q++) {
@@ -606,17 +607,17 @@ __global__ void __launch_bounds__(kFluxThreads) __attribute__((amdgpu_waves_per_
         asm volatile("" : "+v"(ybase), "+v"(zbase), "+v"(hbase));
         if (yact) {
             double F[NV];
-            face_flux_ld<NS>(1, TileLd<FS>{tile, ybase, TC}, F);
+            face_flux_ld<NS, ORDER>(1, TileLd<FS>{tile, ybase, TC}, F);
 #pragma unroll
             for (int q = 0; q < NV; q++)
                 fy[(q * (ROWS + 1) + ys) * KW + ycol] = F[q];
         }
         double Fzp[NV] = {};
         if (own)
-            face_flux_ld<NS>(2, TileLd<FS>{tile, zbase, 1}, Fzp);
+            face_flux_ld<NS, ORDER>(2, TileLd<FS>{tile, zbase, 1}, Fzp);
         if (zhalo) {
             double F[NV];
-            face_flux_ld<NS>(2, TileLd<FS>{tile, hbase, 1}, F);
+            face_flux_ld<NS, ORDER>(2, TileLd<FS>{tile, hbase, 1}, F);
 #pragma unroll
             for (int q = 0; q < NV; q++)
                 zb[q * ROWS + l] = F[q];
@@ -1732,30 +1733,30 @@ __global__ void k_derive_cp(Grid g, const DevMech *__restrict__ mp, const double
         FAIL(EU3D_ERR_ARG, "species count not compiled in (supported: 9)");                       \
     }
 
-template <int NS, int KW>
+template <int NS, int KW, int ORDER>
 static void launch_flux_kw(eu3d_ctx *ctx, int mode, double dt)
 {
     const Grid &g = ctx->g;
     const dim3 grid(flux_blocks(g, KW));
     switch (mode) {
     case 0:
-        k_flux_update<NS, 0, KW><<<grid, kFluxThreads, 0, ctx->stream>>>(g, ctx->prim, ctx->CS, ctx->CS0, ctx->x,
+        k_flux_update<NS, 0, KW, ORDER><<<grid, kFluxThreads, 0, ctx->stream>>>(g, ctx->prim, ctx->CS, ctx->CS0, ctx->x,
                                                                         ctx->y, ctx->z, dt);
         break;
     case 1:
-        k_flux_update<NS, 1, KW><<<grid, kFluxThreads, 0, ctx->stream>>>(g, ctx->prim, ctx->CS, ctx->CS0, ctx->x,
+        k_flux_update<NS, 1, KW, ORDER><<<grid, kFluxThreads, 0, ctx->stream>>>(g, ctx->prim, ctx->CS, ctx->CS0, ctx->x,
                                                                         ctx->y, ctx->z, dt);
         break;
     case 2:
-        k_flux_update<NS, 2, KW><<<grid, kFluxThreads, 0, ctx->stream>>>(g, ctx->prim, ctx->CS, ctx->CS0, ctx->x,
+        k_flux_update<NS, 2, KW, ORDER><<<grid, kFluxThreads, 0, ctx->stream>>>(g, ctx->prim, ctx->CS, ctx->CS0, ctx->x,
                                                                         ctx->y, ctx->z, dt);
         break;
     case 3:
-        k_flux_update<NS, 3, KW><<<grid, kFluxThreads, 0, ctx->stream>>>(g, ctx->prim, ctx->CS, ctx->CS0, ctx->x,
+        k_flux_update<NS, 3, KW, ORDER><<<grid, kFluxThreads, 0, ctx->stream>>>(g, ctx->prim, ctx->CS, ctx->CS0, ctx->x,
                                                                         ctx->y, ctx->z, dt);
         break;
     default:
-        k_flux_update<NS, 4, KW><<<grid, kFluxThreads, 0, ctx->stream>>>(g, ctx->prim, ctx->CS, ctx->CS0, ctx->x,
+        k_flux_update<NS, 4, KW, ORDER><<<grid, kFluxThreads, 0, ctx->stream>>>(g, ctx->prim, ctx->CS, ctx->CS0, ctx->x,
                                                                         ctx->y, ctx->z, dt);
         break;
     }
@@ -1764,20 +1765,29 @@ static void launch_flux_kw(eu3d_ctx *ctx, int mode, double dt)
 // Row width of the flux tiles: 64 cells, or 32 / 16 on grids thinner in k.
 static int flux_row_width(const Grid &g) { return g.nk >= 48 ? 64 : g.nk >= 24 ? 32 : 16; }
 
-template <int NS, class MT>
-static void launch_flux(eu3d_ctx *ctx, int mode, double dt)
+template <int NS, int ORDER>
+static void launch_flux_order(eu3d_ctx *ctx, int mode, double dt)
 {
     switch (flux_row_width(ctx->g)) {
     case 64:
-        launch_flux_kw<NS, 64>(ctx, mode, dt);
+        launch_flux_kw<NS, 64, ORDER>(ctx, mode, dt);
         break;
     case 32:
-        launch_flux_kw<NS, 32>(ctx, mode, dt);
+        launch_flux_kw<NS, 32, ORDER>(ctx, mode, dt);
         break;
     default:
-        launch_flux_kw<NS, 16>(ctx, mode, dt);
+        launch_flux_kw<NS, 16, ORDER>(ctx, mode, dt);
         break;
     }
+}
+
+template <int NS, class MT>
+static void launch_flux(eu3d_ctx *ctx, int mode, double dt)
+{
+    if (ctx->difference == 1)
+        launch_flux_order<NS, 1>(ctx, mode, dt);
+    else
+        launch_flux_order<NS, 2>(ctx, mode, dt);
 }
 
 template <int NS, class MT>
@@ -1950,6 +1960,12 @@ int eu3d_set_option(eu3d_ctx *ctx, int option, int value)
         ctx->generated = ctx->generated_available && value == 0;
         return EU3D_OK;
     }
+    if (option == EU3D_OPT_DIFFERENCE) {
+        if (value != 1 && value != 2)
+            FAIL(EU3D_ERR_ARG, "EU3D_OPT_DIFFERENCE must be 1 (MUSCL_1) or 2 (MUSCL_2)");
+        ctx->difference = value;
+        return EU3D_OK;
+    }
     FAIL(EU3D_ERR_ARG, "unknown option");
 }
 
@@ -1963,6 +1979,10 @@ int eu3d_get_option(eu3d_ctx *ctx, int option, int *value)
     }
     if (option == EU3D_OPT_GENERATED_AVAILABLE) {
         *value = ctx->generated_available ? 1 : 0;
+        return EU3D_OK;
+    }
+    if (option == EU3D_OPT_DIFFERENCE) {
+        *value = ctx->difference;
         return EU3D_OK;
     }
     FAIL(EU3D_ERR_ARG, "unknown option");

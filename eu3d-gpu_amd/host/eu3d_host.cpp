@@ -776,10 +776,19 @@ Euler::Euler(const std::string &ctrlfile, const std::string &base_dir, bool devi
         Reaction_type = 0;
     else if (rs == "IMEX")
         Reaction_type = 2;
+    else if (rs == "DNN" || rs == "Cantera")
+        Reaction_type = 3; // Global::DNN: Euler::DNN (src/Euler.cpp:1473-1480) runs no chemistry
     else
-        throw Error(EU3D_ERR_ARG, "reaction scheme must be Trapezoid, Trapezoid-Adaptive or IMEX");
-    if (Ctrl.getStringParameter("difference") != "MUSCL_2")
-        throw Error(EU3D_ERR_ARG, "difference scheme must be MUSCL_2");
+        throw Error(EU3D_ERR_ARG, "reaction scheme must be Trapezoid, Trapezoid-Adaptive, IMEX, DNN or Cantera");
+    // Euler::FileRead (src/Euler.cpp:162-176); WENO is parsed there but has no
+    // implementation (Flowfield::Advection, src/Flowfield.cpp:1140-1149)
+    const std::string df = Ctrl.getStringParameter("difference");
+    if (df == "MUSCL_1")
+        Diff_Sch = 1;
+    else if (df == "MUSCL_2")
+        Diff_Sch = 2;
+    else
+        throw Error(EU3D_ERR_ARG, "difference scheme must be MUSCL_1 or MUSCL_2");
     std::string ta = Ctrl.getStringParameter("time-adv");
     if (ta == "EE")
         TimeAdv_Sch = EU3D_SCHEME_EE;
@@ -814,6 +823,7 @@ void Euler::Attach()
     }
     if (TwoDim.boundary != EU3D_BOUNDARY_RSBI)
         check(eu3d_set_boundary(TwoDim.ctx, TwoDim.boundary, TwoDim.inflow.data()), TwoDim.ctx, "eu3d_set_boundary");
+    check(eu3d_set_option(TwoDim.ctx, EU3D_OPT_DIFFERENCE, Diff_Sch), TwoDim.ctx, "eu3d_set_option");
     TwoDim.Upload();
 }
 
@@ -842,7 +852,8 @@ void Euler::Computing(int max_steps)
             Output(out_num);
             out_num = out_num + 1;
         }
-        const int flags = chem_flags | (Reaction_type == 0 ? EU3D_CHEM_FIXED : 0);
+        const int flags = chem_flags | (Reaction_type == 0 ? EU3D_CHEM_FIXED : 0) |
+                          (Reaction_type == 3 ? EU3D_CHEM_SKIP : 0);
         const int scheme = Reaction_type == 2 ? EU3D_SCHEME_IMEX : TimeAdv_Sch;
         check(eu3d_step(TwoDim.ctx, scheme, TwoDim.dt, flags), TwoDim.ctx, "eu3d_step");
         time += TwoDim.dt;
@@ -853,7 +864,7 @@ void Euler::Computing(int max_steps)
         // every step; here every 16 steps and at the end of the call, so that
         // the step loop does not wait for the device each step.
         LoadIterations.push_back(iteration);
-        if (!(chem_flags & EU3D_CHEM_SKIP) &&
+        if (!(flags & EU3D_CHEM_SKIP) &&
             ((iteration % 16) == 0 || n == max_steps || Final_Time - time <= time_tol)) {
             int got = 0;
             std::vector<int> mx(1 << 16);

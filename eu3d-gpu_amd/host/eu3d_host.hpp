@@ -153,7 +153,9 @@ class Euler {
     std::string base;
     double Final_Time = 0, cfl = 0, dt = 0;
     int count = 0, TimeAdv_Sch = 0, iteration = 0;
-    int Reaction_type = 1; // ctrl "reaction": Trapezoid = 0, Trapezoid-Adaptive = 1 (src/Euler.cpp:136-160)
+    int Reaction_type = 1; // ctrl "reaction": Trapezoid = 0, Trapezoid-Adaptive = 1, IMEX = 2, DNN/Cantera = 3
+                           // (no chemistry) (src/Euler.cpp:136-160)
+    int Diff_Sch = 2;      // ctrl "difference": MUSCL_1 = 1, MUSCL_2 = 2 (src/Euler.cpp:162-176)
     int chem_flags = EU3D_CHEM_COMPACT | EU3D_CHEM_KEEP_CMS;
     double time = 0;
     Mesh Mymesh;

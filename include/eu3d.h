@@ -172,9 +172,13 @@ int eu3d_health(eu3d_ctx *ctx, eu3d_health_info *out);
  * a mechanism compiled into the library (the compiled form is chosen by
  * default; both give the same bits).  EU3D_OPT_GENERATED_AVAILABLE (read
  * only): 1 if the tables match a compiled-in mechanism.  No reference
- * counterpart (the reference has one code path). */
+ * counterpart (the reference has one code path).
+ * EU3D_OPT_DIFFERENCE (read/write): the face reconstruction of the flux,
+ * 2 = MUSCL_2 with Minmod (default), 1 = MUSCL_1, first order -- the ctrl
+ * keyword `difference` (src/Euler.cpp:162-176, include/Differ.hpp:110-238). */
 #define EU3D_OPT_GENERIC_MECHANISM 1
 #define EU3D_OPT_GENERATED_AVAILABLE 2
+#define EU3D_OPT_DIFFERENCE 3
 int eu3d_set_option(eu3d_ctx *ctx, int option, int value);
 int eu3d_get_option(eu3d_ctx *ctx, int option, int *value);
 

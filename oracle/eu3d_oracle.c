@@ -891,12 +891,19 @@ or_state *or_init(const char *ctrl_path, const char *base_dir, char *err, int er
             s->reaction_type = 0;
         else if (r && strcmp(r, "IMEX") == 0)
             s->reaction_type = 2;
+        else if (r && (strcmp(r, "DNN") == 0 || strcmp(r, "Cantera") == 0))
+            s->reaction_type = 3; /* Global::DNN: Euler::DNN (src/Euler.cpp:1473-1480) runs no chemistry */
         else {
-            snprintf(err, errlen, "reaction scheme must be Trapezoid, Trapezoid-Adaptive or IMEX");
+            snprintf(err, errlen, "reaction scheme must be Trapezoid, Trapezoid-Adaptive, IMEX, DNN or Cantera");
             goto fail;
         }
-        if (!d || strcmp(d, "MUSCL_2") != 0) {
-            snprintf(err, errlen, "difference scheme must be MUSCL_2");
+        /* Euler::FileRead (src/Euler.cpp:162-176), Flowfield::Advection (src/Flowfield.cpp:1140-1149) */
+        if (d && strcmp(d, "MUSCL_1") == 0)
+            s->diff = 0;
+        else if (d && strcmp(d, "MUSCL_2") == 0)
+            s->diff = 1;
+        else {
+            snprintf(err, errlen, "difference scheme must be MUSCL_1 or MUSCL_2");
             goto fail;
         }
         if (t && strcmp(t, "EE") == 0)
@@ -907,7 +914,6 @@ or_state *or_init(const char *ctrl_path, const char *base_dir, char *err, int er
             snprintf(err, errlen, "time-adv must be EE or TVD_RK3");
             goto fail;
         }
-        s->diff = 1;
         if (s->reaction_type == 2 && s->time_adv != 0) {
             snprintf(err, errlen, "IMEX needs time-adv EE (it adds the EE right-hand side itself)");
             goto fail;
@@ -1222,6 +1228,25 @@ void or_muscl2(int direction, int NI, int NJ, int NK, int bc, const double *fi, 
             }
 }
 
+/* MUSCL_1 (include/Differ.hpp:110-137): first order, the left state of the
+ * face between i and i+1 is f(i), the right state f(i+1). */
+void or_muscl1(int direction, int NI, int NJ, int NK, int bc, const double *fi, double *FLR)
+{
+    int ni = NI - 2 * bc, nj = NJ - 2 * bc, nk = NK - 2 * bc;
+    size_t si = (size_t)NJ * NK, sj = (size_t)NK, sk = 1;
+    size_t st = direction == 1 ? si : (direction == 2 ? sj : sk);
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static)
+#endif
+    for (int i = bc - 1; i < ni + bc; i++)
+        for (int j = bc - 1; j < nj + bc; j++)
+            for (int k = bc - 1; k < nk + bc; k++) {
+                size_t id = i * si + j * sj + k * sk;
+                FLR[id * 2 + 0] = fi[id];
+                FLR[id * 2 + 1] = fi[id + st];
+            }
+}
+
 /* ====================================================================== */
 /* AUSM (src/Flowfield.cpp:1757-1931)                                     */
 /* ====================================================================== */
@@ -1290,13 +1315,14 @@ static void ausm_dir(or_state *s, int direction, double *Fi)
     int nf = 7 + ns;
     size_t N = (size_t)s->NI * s->NJ * s->NK;
     double *fields[7] = {s->P, s->D, s->U, s->V, s->W, s->H, s->Gamma};
+    void (*diff)(int, int, int, int, int, const double *, double *) = s->diff == 0 ? or_muscl1 : or_muscl2;
     for (int f = 0; f < 7; f++)
-        or_muscl2(direction, s->NI, s->NJ, s->NK, bc, fields[f], s->FLR + (size_t)f * 2 * N);
+        diff(direction, s->NI, s->NJ, s->NK, bc, fields[f], s->FLR + (size_t)f * 2 * N);
     double *tmp = dalloc(N);
     for (int q = 0; q < ns; q++) {
         for (size_t id = 0; id < N; id++)
             tmp[id] = s->Yi[id * ns + q];
-        or_muscl2(direction, s->NI, s->NJ, s->NK, bc, tmp, s->FLR + (size_t)(7 + q) * 2 * N);
+        diff(direction, s->NI, s->NJ, s->NK, bc, tmp, s->FLR + (size_t)(7 + q) * 2 * N);
     }
     free(tmp);
     /* F.Fill(0) (src/Flowfield.cpp:1159-1163) */
@@ -1747,8 +1773,9 @@ void or_step(or_state *s, double dt)
     }
     if (s->reaction_type == 0)
         or_chemistry_fixed(s);
-    else
+    else if (s->reaction_type == 1)
         or_chemistry(s);
+    /* reaction_type 3 (DNN / Cantera): no chemistry, as Euler::DNN */
     s->time += dt;
     s->iteration += 1;
 }

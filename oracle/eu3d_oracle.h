@@ -53,8 +53,8 @@ typedef struct or_state {
     double final_time, cfl, dt_fixed;
     int count;
     int time_adv;                /* 0 = EE, 1 = TVD_RK3 */
-    int diff;                    /* 1 = MUSCL_2 (only supported scheme) */
-    int reaction_type;           /* 0 = Trapezoid, 1 = Trapezoid-Adaptive, 2 = IMEX */
+    int diff;                    /* 0 = MUSCL_1, 1 = MUSCL_2 */
+    int reaction_type;           /* 0 = Trapezoid, 1 = Trapezoid-Adaptive, 2 = IMEX, 3 = DNN/Cantera (none) */
 
     /* flow field (include/Flowfield.hpp:68-110) */
     double *U, *V, *W, *P, *D, *T, *C, *Ma, *Wav, *Rgas, *Cp, *H, *E, *Gamma;
@@ -83,6 +83,7 @@ int or_num_threads(int n);     /* set OpenMP threads (if built with OpenMP); ret
 /* ---- the step and its pieces ---------------------------------------- */
 double or_cfl(or_state *s, double cfl, double final_time);       /* Flowfield::CFLcondition */
 void or_field_boundary(or_state *s);                              /* FieldBoundary_3dRSBI */
+void or_muscl1(int direction, int NI, int NJ, int NK, int bc, const double *fi, double *FLR); /* MUSCL_1 */
 void or_advection(or_state *s);                                   /* Advection(EE, MUSCL_2) -> RHS */
 void or_update_after_adv(or_state *s);                            /* CS += RHS; Update_after_CS() */
 void or_update_after_cs(or_state *s);                             /* Update_after_CS() */

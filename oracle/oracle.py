@@ -35,6 +35,8 @@ _SIGS = {
     "or_minmod": (ctypes.c_double, [ctypes.c_double]),
     "or_muscl2": (None, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                          ctypes.c_void_p, ctypes.c_void_p]),
+    "or_muscl1": (None, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                         ctypes.c_void_p, ctypes.c_void_p]),
     "or_ausm_face": (None, [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]),
     "or_get_temp": (ctypes.c_double, [ctypes.c_void_p, ctypes.c_double, ctypes.c_double, ctypes.c_void_p]),
     "or_rates": (None, [ctypes.c_void_p, ctypes.c_double, ctypes.c_void_p] + [ctypes.c_void_p] * 7),

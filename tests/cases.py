@@ -55,7 +55,7 @@ phi             1.0
 
 
 def write_case(tmpdir, name, ni, nj, nk, extents, input_text, time_adv="EE", dt=3e-8, cfl=0.0,
-               final_time=None, steps=1, reaction="Trapezoid-Adaptive"):
+               final_time=None, steps=1, reaction="Trapezoid-Adaptive", difference="MUSCL_2"):
     """Write ctrl-<name>.txt (+ grid, input) into tmpdir; return the ctrl path."""
     xa, xb, ya, yb, za, zb = extents
     os.makedirs(tmpdir, exist_ok=True)
@@ -74,27 +74,27 @@ def write_case(tmpdir, name, ni, nj, nk, extents, input_text, time_adv="EE", dt=
                 f"reaction-model {os.path.join(INPUT, 'H2_JA919.txt')}\n"
                 f"Thermofile {os.path.join(INPUT, 'thermo.dat')}\n"
                 f"Final-Time {final_time:.12g}\ncfl {cfl!r}\ndt {dt!r}\ncount 1\n"
-                f"difference MUSCL_2\ntime-adv {time_adv}\nreaction {reaction}\n"
+                f"difference {difference}\ntime-adv {time_adv}\nreaction {reaction}\n"
                 f"m_block_x 1\nm_block_y 1\nm_block_z 1\nnum_thread 1\nDLB_step 5\nDLB_tol 0.001\n")
     return ctrl
 
 
 def rsbi(tmpdir, ni=17, nj=13, nk=4, steps=3, time_adv="EE", T_pre=1000.0, M_wave=3.0, dt=3e-8,
-         reaction="Trapezoid-Adaptive"):
+         reaction="Trapezoid-Adaptive", difference="MUSCL_2"):
     """The reference's runnable problem (3dRSBI) on a reduced grid, same extents."""
     return write_case(tmpdir, f"rsbi{ni}x{nj}x{nk}", ni, nj, nk, (0.0, 0.17, 0.0, 0.08, 0.0, 0.02),
                       RSBI_INPUT.format(T_pre=T_pre, M_wave=M_wave), time_adv=time_adv, dt=dt, steps=steps,
-                      reaction=reaction)
+                      reaction=reaction, difference=difference)
 
 
 def sbi(tmpdir, ni=32, nj=16, nk=16, h=1e-4, gap=3, seed=20240601, time_adv="TVD_RK3", cfl=0.5, r_frac=0.2,
-        reaction="Trapezoid-Adaptive"):
+        reaction="Trapezoid-Adaptive", difference="MUSCL_2"):
     """Spherical shock-bubble (3dSBI) on a reduced grid with spacing h
     (a large negative r_frac leaves no bubble gas: pure nitrogen)."""
     return write_case(tmpdir, f"sbi{ni}x{nj}x{nk}", ni, nj, nk,
                       (0.0, (ni - 1) * h, 0.0, (nj - 1) * h, 0.0, (nk - 1) * h),
                       SBI_INPUT.format(gap=gap, seed=seed, r_frac=r_frac), time_adv=time_adv, cfl=cfl, dt=0.0,
-                      reaction=reaction)
+                      reaction=reaction, difference=difference)
 
 
 def znd(tmpdir, ni=201, time_adv="TVD_RK3", cfl=0.5, length=0.01):

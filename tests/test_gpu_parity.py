@@ -256,6 +256,48 @@ def test_flux_row_widths(tmp_path, nk, nj):
     compare_state(o, s, 1e-10, f"SBI RK3 3 steps nk {nk}")
 
 
+@pytest.mark.parametrize("steps,tol", [(1, 1e-12), (100, 1e-8)])
+def test_rsbi_muscl1(tmp_path, steps, tol):
+    """ctrl `difference MUSCL_1` (first-order face states, include/Differ.hpp:
+    110-137) on the RSBI problem, EE + adaptive chemistry."""
+    note(f"rsbi MUSCL_1 {steps} steps")
+    o, s = pair(cases.rsbi(str(tmp_path), steps=steps, difference="MUSCL_1"))
+    assert o.run(steps) == steps
+    s.run(steps)
+    s.sync()
+    compare_state(o, s, tol, f"RSBI MUSCL_1 {steps} steps")
+
+
+@pytest.mark.parametrize("nk", [70, 40, 16])
+def test_sbi_muscl1_row_widths(tmp_path, nk):
+    """MUSCL_1 through the flux tiles of every row width, RK3."""
+    note(f"sbi MUSCL_1 nk {nk}")
+    o, s = pair(cases.sbi(str(tmp_path), ni=12, nj=10, nk=nk, difference="MUSCL_1"))
+    for step in range(3):
+        o.step(o.cfl(0.5, 1.0))
+        s.run(1)
+    s.sync()
+    compare_state(o, s, 1e-10, f"SBI MUSCL_1 3 steps nk {nk}")
+
+
+@pytest.mark.parametrize("reaction", ["DNN", "Cantera"])
+def test_reaction_dnn_runs_no_chemistry(tmp_path, reaction):
+    """ctrl `reaction DNN` / `Cantera`: the reference's Euler::DNN
+    (src/Euler.cpp:1473-1480) runs no chemistry, so only the flow advances;
+    the hot RSBI case reacts otherwise."""
+    note(f"rsbi reaction {reaction}")
+    ctrl = cases.rsbi(str(tmp_path), steps=10, T_pre=1200.0, dt=3e-7, reaction=reaction)
+    o, s = pair(ctrl)
+    assert o.run(10) == 10
+    s.run(10)
+    s.sync()
+    compare_state(o, s, 1e-10, f"RSBI reaction {reaction} 10 steps")
+    r = eu3d.Solver(cases.rsbi(str(tmp_path / "r"), steps=10, T_pre=1200.0, dt=3e-7))
+    r.run(10)
+    r.sync()
+    assert np.abs(r.interior(r.field("Yi")) - s.interior(s.field("Yi"))).max() > 1e-6
+
+
 def test_znd_rk3(tmp_path):
     note("znd rk3 30 steps")
     ctrl = cases.znd(str(tmp_path), ni=201)

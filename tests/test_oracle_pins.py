@@ -409,3 +409,21 @@ def test_imex_is_ee_update_for_an_inert_mixture(tmp_path):
     for n in ["CS", "T", "P", "D", "U", "Yi"]:
         assert np.array_equal(_bits(a.interior(a.field(n))), _bits(b.interior(b.field(n)))), n
     assert np.all(b.interior(b.field("CMS")) == 0.0)
+
+
+def test_muscl1_is_first_order_copy():
+    """MUSCL_1 (include/Differ.hpp:110-137): the face between i and i+1 gets
+    f(i) on the left and f(i+1) on the right, in every direction, over the
+    faces bc-1 .. n+bc-1 of each axis (brute force on a small random field)."""
+    NI, NJ, NK, bc = 7, 6, 5, 2
+    rng = np.random.default_rng(3)
+    f = rng.standard_normal(NI * NJ * NK)
+    F = f.reshape(NI, NJ, NK)
+    for d in (1, 2, 3):
+        out = np.full(2 * f.size, np.nan)
+        oracle.lib().or_muscl1(d, NI, NJ, NK, bc, oracle.ptr(f), oracle.ptr(out))
+        o = out.reshape(NI, NJ, NK, 2)
+        sl = tuple(slice(bc - 1, n - bc) for n in (NI, NJ, NK))
+        nxt = [np.roll(F, -1, axis=a) for a in range(3)][d - 1]
+        assert np.array_equal(o[sl + (0,)], F[sl])
+        assert np.array_equal(o[sl + (1,)], nxt[sl])
