@@ -33,6 +33,7 @@ BOUNDARY_RSBI = 0
 BOUNDARY_ODW = 1
 OPT_GENERIC_MECHANISM = 1
 OPT_GENERATED_AVAILABLE = 2
+OPT_DIFFERENCE = 3
 
 _vp = ctypes.c_void_p
 _i = ctypes.c_int
