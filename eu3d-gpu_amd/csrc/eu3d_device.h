@@ -28,6 +28,12 @@ struct DevMech {
     double nf[kMaxNS * kMaxNR], nb[kMaxNS * kMaxNR], tb[kMaxNS * kMaxNR]; // s*nr + r
     double p0x[kMaxNR]; // pow(P0*1e-6, Xr), evaluated on the host with the C library
     double mxr[kMaxNR]; // -Xr, Xr = sum_s (nb - nf) in species order
+    // Temperatures [dead_tlo, dead_thi] in which every rate constant, its
+    // equilibrium constant and its reverse are finite (checked on the host
+    // for the tables above; empty when the check fails).  There a reaction
+    // whose forward and reverse concentration products are both zero adds
+    // exact zeros to every sum, so its rate constants are not evaluated.
+    double dead_tlo, dead_thi;
 
     __device__ __forceinline__ const double *thermo_low() const { return c0; }
     __device__ __forceinline__ const double *thermo_high() const { return c1; }
@@ -413,29 +419,62 @@ struct Rates {
     double P[NS]; // trapezoid loss term, before the division by Mc
 };
 
-// One reaction's contribution: rate constants, rates of progress and the
-// per-species sums (src/Reaction.cpp:609-631, 655-662, 781-793).
-template <int NS, bool want_pq, class MT>
-__device__ __forceinline__ void reaction_term(const MT &m, int r, double T, const double (&Gi)[NS],
-                                              const double (&Mc)[NS], Rates<NS> &o)
+// Forward and reverse products of concentrations of reaction r
+// (src/Reaction.cpp:617-623), and the third-body sum.
+template <int NS, class MT>
+__device__ __forceinline__ void reaction_products(const MT &m, int r, const double (&Mc)[NS], double &RRF,
+                                                  double &RRB, double &RTB)
 {
     const int nr = m.nr;
-    const double R = m.R, Ru = m.Ru;
-    double KF = m.af[r] * pow_small(T, m.bf[r]);
-    if (m.eaf[r] != 0.0)
-        KF = KF * exp(-m.eaf[r] / (Ru * T));
-    double delta_g = 0.0, RRF = 1.0, RRB = 1.0, RTB = 0.0;
+    RRF = 1.0;
+    RRB = 1.0;
+    RTB = 0.0;
 #pragma unroll
     for (int s = 0; s < NS; s++) {
         const double nb = m.nb[s * nr + r], nf = m.nf[s * nr + r], tb = m.tb[s * nr + r];
-        if (nb != nf)
-            delta_g += (nb - nf) * Gi[s];
         if (nf != 0.0)
             RRF *= pow_small(Mc[s], nf);
         if (nb != 0.0)
             RRB *= pow_small(Mc[s], nb);
         if (tb != 0.0)
             RTB += Mc[s] * tb;
+    }
+}
+
+// The temperature window of the dead-reaction skip (DevMech::dead_tlo/thi).
+template <class MT>
+__device__ __forceinline__ bool dead_window(const MT &m, double T)
+{
+    if constexpr (MT::kGenerated)
+        return T >= m.dev->dead_tlo && T <= m.dev->dead_thi;
+    else
+        return T >= m.dead_tlo && T <= m.dead_thi;
+}
+
+// One reaction's contribution: rate constants, rates of progress and the
+// per-species sums (src/Reaction.cpp:609-631, 655-662, 781-793).  With win
+// (T inside the dead-reaction window) a reaction with RRF = RRB = 0 is
+// skipped: KF * 0 - KB * 0 and the P and Q terms are +-0 there, and adding
+// a signed zero leaves each sum's bits unchanged (the sums start at +0).
+template <int NS, bool want_pq, class MT>
+__device__ __forceinline__ void reaction_term(const MT &m, int r, double T, const double (&Gi)[NS],
+                                              const double (&Mc)[NS], bool win, Rates<NS> &o)
+{
+    const int nr = m.nr;
+    const double R = m.R, Ru = m.Ru;
+    double RRF, RRB, RTB;
+    reaction_products<NS>(m, r, Mc, RRF, RRB, RTB);
+    if (win && RRF == 0.0 && RRB == 0.0)
+        return;
+    double KF = m.af[r] * pow_small(T, m.bf[r]);
+    if (m.eaf[r] != 0.0)
+        KF = KF * exp(-m.eaf[r] / (Ru * T));
+    double delta_g = 0.0;
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+        const double nb = m.nb[s * nr + r], nf = m.nf[s * nr + r];
+        if (nb != nf)
+            delta_g += (nb - nf) * Gi[s];
     }
     double Kp = exp(-delta_g / (R * T)) * m.p0x[r];
     double Kc = Kp * pow_small(R * T, m.mxr[r]);
@@ -457,11 +496,43 @@ __device__ __forceinline__ void reaction_term(const MT &m, int r, double T, cons
 
 // Mc in mol/cm^3.  If want_pq, Q and P sums are formed as well.  With a
 // generated mechanism type the reaction loop is unrolled and every table
-// entry is a compile-time constant.
+// entry is a compile-time constant.  A cell inside the dead-reaction window
+// whose reactions all have zero forward and reverse products (an inert
+// mixture) gets the exact zeros without the Gibbs energies or any rate
+// constant.
 template <int NS, bool want_pq, class MT>
 __device__ __forceinline__ void rates(const MT &m, double T, const double (&Mc)[NS], Rates<NS> &o)
 {
     const double R = m.R;
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+        o.W[s] = 0.0;
+        if (want_pq) {
+            o.Q[s] = 0.0;
+            o.P[s] = 0.0;
+        }
+    }
+    const bool win = dead_window(m, T);
+    bool live = !win;
+    if (win) {
+        if constexpr (MT::kGenerated) {
+#pragma unroll
+            for (int r = 0; r < MT::nr; r++) {
+                double RRF, RRB, RTB;
+                reaction_products<NS>(m, r, Mc, RRF, RRB, RTB);
+                live = live || RRF != 0.0 || RRB != 0.0;
+            }
+        } else {
+#pragma unroll 1
+            for (int r = 0; r < m.nr; r++) {
+                double RRF, RRB, RTB;
+                reaction_products<NS>(m, r, Mc, RRF, RRB, RTB);
+                live = live || RRF != 0.0 || RRB != 0.0;
+            }
+        }
+    }
+    if (!live)
+        return; // W = mw * 0 = +0, as below
     double Gi[NS];
     {
         TPow t = tpow(T);
@@ -474,22 +545,14 @@ __device__ __forceinline__ void rates(const MT &m, double T, const double (&Mc)[
             }
         });
     }
-#pragma unroll
-    for (int s = 0; s < NS; s++) {
-        o.W[s] = 0.0;
-        if (want_pq) {
-            o.Q[s] = 0.0;
-            o.P[s] = 0.0;
-        }
-    }
     if constexpr (MT::kGenerated) {
 #pragma unroll
         for (int r = 0; r < MT::nr; r++)
-            reaction_term<NS, want_pq>(m, r, T, Gi, Mc, o);
+            reaction_term<NS, want_pq>(m, r, T, Gi, Mc, win, o);
     } else {
 #pragma unroll 1
         for (int r = 0; r < m.nr; r++)
-            reaction_term<NS, want_pq>(m, r, T, Gi, Mc, o);
+            reaction_term<NS, want_pq>(m, r, T, Gi, Mc, win, o);
     }
 #pragma unroll
     for (int s = 0; s < NS; s++)

@@ -1947,6 +1947,56 @@ static bool tables_equal(const DevMech &m)
 
 static bool mech_matches_generated(const DevMech &m) { return tables_equal<MechH2JA919>(m); }
 
+// Window of the dead-reaction skip (DevMech::dead_tlo/thi): [200 K, 6000 K]
+// if, on a 0.25 K grid over it, the logarithm of every factor and partial
+// product of every reaction's KF, Kp, Kc and KB stays within +-600 (exp
+// overflows above 709.78 and underflows below -745) and moves by less than 10
+// between neighbouring points, so that no value between the points reaches
+// the limits; otherwise the window is empty and every rate constant is
+// evaluated.  Thermo as Reaction::GetHi/GetSi (src/Reaction.cpp:1184-1220).
+static void set_dead_window(DevMech &m)
+{
+    const double lo = 200.0, hi = 6000.0, step = 0.25, lim = 600.0, jump = 10.0;
+    const int ns = m.ns, nr = m.nr;
+    const int nq = 8;
+    std::vector<double> prev((size_t)nr * nq, 0.0);
+    bool ok = true;
+    for (int n = 0; ok && lo + n * step <= hi; n++) {
+        const double T = lo + n * step, lnT = std::log(T);
+        const double *c = T < EU3D_THERMO_SWITCH_T ? m.c0 : m.c1;
+        double G[kMaxNS];
+        for (int s = 0; s < ns; s++) {
+            auto a = [&](int k) { return c[k * ns + s]; };
+            double h = m.R * (-a(0) / T + a(1) * lnT + a(2) * T + a(3) * T * T / 2 + a(4) * T * T * T / 3 +
+                              a(5) * std::pow(T, 4) / 4 + a(6) * std::pow(T, 5) / 5 + a(7));
+            double sv = m.R * (-a(0) / (T * T) / 2 - a(1) / T + a(2) * lnT + a(3) * T + a(4) * T * T / 2 +
+                               a(5) * T * T * T / 3 + a(6) * std::pow(T, 4) / 4 + a(8));
+            G[s] = h - sv * T;
+        }
+        for (int r = 0; ok && r < nr; r++) {
+            double dg = 0.0;
+            for (int s = 0; s < ns; s++)
+                dg += (m.nb[s * nr + r] - m.nf[s * nr + r]) * G[s];
+            const double lnA = m.af[r] != 0.0 ? std::log(std::fabs(m.af[r])) + m.bf[r] * lnT : 0.0;
+            const double ea = m.eaf[r] / (m.Ru * T);
+            const double lnKF = m.af[r] != 0.0 ? lnA - ea : 0.0;
+            const double gp = -dg / (m.R * T);
+            const double lnKp = gp + std::log(m.p0x[r]);
+            const double rt = m.mxr[r] * std::log(m.R * T);
+            const double lnKc = lnKp + rt;
+            const double q[nq] = {lnA, ea, lnKF, gp, lnKp, rt, lnKc, lnKF - lnKc};
+            for (int k = 0; k < nq; k++) {
+                double &p = prev[(size_t)r * nq + k];
+                if (!(std::fabs(q[k]) < lim) || (n > 0 && !(std::fabs(q[k] - p) < jump)))
+                    ok = false;
+                p = q[k];
+            }
+        }
+    }
+    m.dead_tlo = ok ? lo : 1.0;
+    m.dead_thi = ok ? hi : 0.0;
+}
+
 // ---------------------------------------------------------------------------
 // C ABI
 // ---------------------------------------------------------------------------
@@ -1983,6 +2033,10 @@ int eu3d_get_option(eu3d_ctx *ctx, int option, int *value)
     }
     if (option == EU3D_OPT_DIFFERENCE) {
         *value = ctx->difference;
+        return EU3D_OK;
+    }
+    if (option == EU3D_OPT_DEAD_REACTION_SKIP) {
+        *value = ctx->hmech.dead_tlo <= ctx->hmech.dead_thi ? 1 : 0;
         return EU3D_OK;
     }
     FAIL(EU3D_ERR_ARG, "unknown option");
@@ -2082,6 +2136,7 @@ int eu3d_create(const eu3d_mesh *mesh, const eu3d_mechanism *mc, eu3d_ctx **out)
         CK(hipGetDevice(&dev));
         CK(hipDeviceGetAttribute(&ctx->ncu, hipDeviceAttributeMultiprocessorCount, dev));
     }
+    set_dead_window(m);
     ctx->generated = mech_matches_generated(m);
     ctx->generated_available = ctx->generated;
     CK(hipStreamCreate(&ctx->stream));

@@ -175,10 +175,16 @@ int eu3d_health(eu3d_ctx *ctx, eu3d_health_info *out);
  * counterpart (the reference has one code path).
  * EU3D_OPT_DIFFERENCE (read/write): the face reconstruction of the flux,
  * 2 = MUSCL_2 with Minmod (default), 1 = MUSCL_1, first order -- the ctrl
- * keyword `difference` (src/Euler.cpp:162-176, include/Differ.hpp:110-238). */
+ * keyword `difference` (src/Euler.cpp:162-176, include/Differ.hpp:110-238).
+ * EU3D_OPT_DEAD_REACTION_SKIP (read only): 1 if every rate constant of the
+ * mechanism was found finite over 200-6000 K at eu3d_create; then, at those
+ * temperatures, reactions whose forward and reverse concentration products
+ * are both zero are not evaluated (they add exact zeros).  No reference
+ * counterpart. */
 #define EU3D_OPT_GENERIC_MECHANISM 1
 #define EU3D_OPT_GENERATED_AVAILABLE 2
 #define EU3D_OPT_DIFFERENCE 3
+#define EU3D_OPT_DEAD_REACTION_SKIP 4
 int eu3d_set_option(eu3d_ctx *ctx, int option, int value);
 int eu3d_get_option(eu3d_ctx *ctx, int option, int *value);
 

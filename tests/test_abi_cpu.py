@@ -187,3 +187,21 @@ def test_ctypes_structs_match_the_headers(tmp_path):
         assert got[(cname, ".")] == ctypes.sizeof(cls), cname
         for f in cls._fields_:
             assert got[(cname, f[0])] == getattr(cls, f[0]).offset, (cname, f[0])
+
+
+def test_python_constants_match_the_headers():
+    """Every EU3D_* integer #define of include/*.h that the Python binding
+    mirrors (same name without the prefix) has the same value there."""
+    import glob
+
+    seen = 0
+    for h in sorted(glob.glob(os.path.join(REPO, "include", "*.h"))):
+        for name, val in re.findall(r"^#define EU3D_([A-Z0-9_]+) \(?(-?(?:0x[0-9A-Fa-f]+|\d+))\)?", open(h).read(), re.M):
+            py = getattr(eu3d, name, None)
+            if py is None:
+                py = getattr(eu3d, "EU3D_" + name, None)
+            if py is None:
+                continue
+            assert py == int(val, 0), (h, name, py, val)
+            seen += 1
+    assert seen >= 15

@@ -147,6 +147,58 @@ def test_thermo_probe(tmp_path):
                 assert abs(a - b) <= 1e-13 * (1.0 + abs(b)), (T[t], q, a, b)
 
 
+@pytest.mark.parametrize("generic", [0, 1])
+def test_rates_probe_dead_reactions(tmp_path, generic):
+    """Production rates on concentration sets where some or all reactions
+    have zero forward and reverse products (inert N2, air, fuel without
+    oxidizer, radical-free premix, signed zeros, one trace radical) and on a
+    full set, at temperatures inside and outside the window in which such
+    reactions are skipped, with the compile-time (generic 0) and the run-time
+    mechanism tables.  Where the oracle's rate is exactly zero the device's has
+    the same bits; elsewhere they agree to 1e-13 of the cell's largest rate."""
+    o, s = pair(cases.rsbi(str(tmp_path)))
+    L = eu3d.load()
+    eu3d.check(L.eu3d_set_option(s.ctx, eu3d.OPT_GENERIC_MECHANISM, generic), s.ctx)
+    v = ctypes.c_int(0)
+    eu3d.check(L.eu3d_get_option(s.ctx, eu3d.OPT_DEAD_REACTION_SKIP, ctypes.byref(v)), s.ctx)
+    assert v.value == 1  # the bundled mechanism passes the finiteness check
+    ns = o.ns
+    mw = o.field("mw")
+    sp = {name: int(np.argmin(np.abs(mw - w))) for name, w in [("N2", 28.0134), ("O2", 31.9988), ("H2", 2.01588),
+                                                                ("H", 1.00794), ("OH", 17.00734)]}
+    assert len(set(sp.values())) == len(sp)
+
+    def mc(**kw):
+        m = np.zeros(ns)
+        for k, v in kw.items():
+            m[sp[k]] = v
+        return m
+
+    rng = np.random.default_rng(3)
+    sets = [mc(N2=3e-5), mc(O2=8e-6, N2=3e-5), mc(H2=1e-5, N2=3e-5), mc(H2=1e-5, O2=5e-6, N2=3e-5),
+            mc(H2=1e-5, O2=5e-6, N2=3e-5, H=-0.0, OH=-0.0), mc(N2=3e-5, H=1e-300), 10.0 ** rng.uniform(-9, -5, ns)]
+    temps = [150.0, 199.9, 200.0, 300.0, 999.9, 1000.0, 2500.0, 6000.0, 6000.1, 7000.0]
+    T = np.array([t for t in temps for _ in sets])
+    M = np.ascontiguousarray(np.array([m for _ in temps for m in sets]))
+    n = len(T)
+    wi = np.zeros(n * ns)
+    dummy = np.zeros(n * 24)
+    eu3d.check(L.eu3d_probe_rates(s.ctx, eu3d.ptr(T), eu3d.ptr(M), n, eu3d.ptr(dummy), eu3d.ptr(dummy),
+                                  eu3d.ptr(dummy), eu3d.ptr(wi)), s.ctx)
+    wi = wi.reshape(n, ns)
+    zeros = 0
+    for c in range(n):
+        ref = o.rates(T[c], M[c])["wi"]
+        scale = np.abs(ref).max()
+        for q in range(ns):
+            if ref[q] == 0.0:
+                zeros += 1
+                assert np.float64(wi[c, q]).view(np.int64) == np.float64(ref[q]).view(np.int64), (T[c], c, q, wi[c, q])
+            else:
+                assert abs(wi[c, q] - ref[q]) <= 1e-13 * scale, (T[c], c, q, wi[c, q], ref[q])
+    assert zeros >= len(temps) * ns  # the inert set alone is all zeros
+
+
 @pytest.mark.parametrize("direction", [1, 2, 3])
 def test_face_flux_matches_oracle(tmp_path, direction):
     """MUSCL_2 + AUSM face fluxes of one direction from the same ghost-filled state."""
