@@ -109,14 +109,45 @@ struct CoefPtr {
     __device__ __forceinline__ double operator()(int idx) const { return a[idx]; }
 };
 
+// NASA-9 coefficients staged in a block's LDS: both ranges, the low one at
+// a[0 .. 9 ns), the high one after it; off selects the range.
+struct CoefLds {
+    const double *a;
+    int off;
+    __device__ __forceinline__ double operator()(int idx) const { return a[off + idx]; }
+};
+
+// A generated mechanism whose NASA-9 tables are read from the block's LDS
+// copy th (see CoefLds) instead of device memory.
+template <class MT>
+struct LdsThermo : MT {
+    const double *th;
+};
+template <class M>
+struct IsLdsThermo {
+    static constexpr bool value = false;
+};
+template <class MT>
+struct IsLdsThermo<LdsThermo<MT>> {
+    static constexpr bool value = true;
+};
+
 // Calls f(coefficients) for the temperature range of T (T < 1000 K: low,
 // include/ThermoConstants.hpp:3).  Each lane reads its own range's table
-// with vector loads: uniform (scalar) operands would be hoisted out of the
-// Newton and sub-step loops by the compiler and spilled.
+// with vector (or LDS) loads: uniform (scalar) operands would be hoisted out
+// of the Newton and sub-step loops by the compiler and spilled.
 template <class MT, class F>
 __device__ __forceinline__ void by_range(const MT &m, double T, F &&f)
 {
     const bool low = T < EU3D_THERMO_SWITCH_T;
+    if constexpr (IsLdsThermo<MT>::value) {
+        // the offset is opaque, so the loads stay at their use (the LDS
+        // base stays visible: ds_read, not flat loads)
+        int off = low ? 0 : 9 * MT::ns;
+        asm volatile("" : "+v"(off));
+        f(CoefLds{m.th, off});
+        return;
+    }
     const double *p = low ? m.thermo_low() : m.thermo_high();
     // keep the table loads at their use: hoisted out of the Newton loop
     // they would hold 162 values in registers for the whole loop

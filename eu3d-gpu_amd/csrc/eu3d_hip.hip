@@ -218,6 +218,35 @@ __device__ __forceinline__ decltype(auto) mech_ref(const DevMech *p)
         return static_cast<const DevMech &>(*p);
 }
 
+// Copies both NASA-9 ranges (9 ns values each) into the block's LDS; every
+// thread of the block calls it (it ends with a barrier).
+template <int NS>
+__device__ __forceinline__ void stage_thermo(const DevMech *__restrict__ mp, double *th)
+{
+    for (int q = threadIdx.x; q < 18 * NS; q += blockDim.x)
+        th[q] = q < 9 * NS ? mp->c0[q] : mp->c1[q - 9 * NS];
+    __syncthreads();
+}
+
+// The mechanism as seen by the Newton-heavy kernels: a generated mechanism
+// reads its NASA-9 tables from the block's LDS copy th (LDS latency instead
+// of a vector-memory round trip per Newton iteration); the run-time tables
+// are read from device memory.
+template <class MT>
+__device__ __forceinline__ decltype(auto) mech_view(const DevMech *p, const double *th)
+{
+    if constexpr (MT::kGenerated)
+        return LdsThermo<MT>{MT{p}, th};
+    else
+        return static_cast<const DevMech &>(*p);
+}
+
+// LDS of a mechanism view: the two NASA-9 ranges for a generated mechanism.
+#define EU3D_THERMO_LDS_STAGE(NS, MT, mp)                                                              \
+    __shared__ double eu3d_th[MT::kGenerated ? 18 * (NS) : 1];                                      \
+    if constexpr (MT::kGenerated)                                                                   \
+        stage_thermo<NS>(mp, eu3d_th);
+
 // ---------------------------------------------------------------------------
 // ghost fill (src/Flowfield.cpp:593-807)
 // ---------------------------------------------------------------------------
@@ -696,6 +725,7 @@ __global__ void __launch_bounds__(kBlock)
     k_recover(Grid g, const DevMech *__restrict__ mp, const double *__restrict__ CS, double *__restrict__ prim,
               double *__restrict__ T, double *__restrict__ E, double *__restrict__ C)
 {
+    EU3D_THERMO_LDS_STAGE(NS, MT, mp)
     long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
     int i, j, k;
     if (!interior_cell(g, t, i, j, k))
@@ -706,7 +736,7 @@ __global__ void __launch_bounds__(kBlock)
     for (int q = 0; q < NS + 4; q++)
         cs[q] = CS[q * N + c];
     CellState<NS> st;
-    recover<NS>(mech_ref<MT>(mp), cs, T[c], E[c], st);
+    recover<NS>(mech_view<MT>(mp, eu3d_th), cs, T[c], E[c], st);
     long long cw = c; // store addresses formed here, not held through the Newton
     asm volatile("" : "+v"(cw) : "v"(st.T));
     store_state<NS>(N, cw, st, prim, T, E, C);
@@ -934,7 +964,8 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(EU3
                  double *__restrict__ CMS, int *__restrict__ nchem, int *__restrict__ queue, ChemCounters *cnt,
                  int compact, int keep_cms, int sur_min, int *__restrict__ sur_list, double sur_tmin = 0.0)
 {
-    const auto &m = mech_ref<MT>(mp);
+    EU3D_THERMO_LDS_STAGE(NS, MT, mp)
+    const auto &m = mech_view<MT>(mp, eu3d_th);
     long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
     int i = 0, j = 0, k = 0;
     bool active = interior_cell(g, t, i, j, k);
@@ -1099,8 +1130,9 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(EU3
                  double *__restrict__ CMS, const int *__restrict__ nchem, const int *__restrict__ queue,
                  ChemCounters *cnt, int keep_cms)
 {
+    EU3D_THERMO_LDS_STAGE(NS, MT, mp)
     const unsigned int total = cnt->queue_count;
-    const auto &m = mech_ref<MT>(mp);
+    const auto &m = mech_view<MT>(mp, eu3d_th);
     long long c = -1;
     int n = 0, step = 0; // the cell is done when step >= n
     bool fresh = false;  // FB: sub-step 0 (with the Nchem prediction) still to do
@@ -1305,12 +1337,13 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(EU3
                  double *__restrict__ prim, double *__restrict__ T, double *__restrict__ E, double *__restrict__ C,
                  double *__restrict__ CMS, int *__restrict__ nchem, ChemCounters *cnt, int keep_cms)
 {
+    EU3D_THERMO_LDS_STAGE(NS, MT, mp)
     long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
     int i = 0, j = 0, k = 0;
     const bool active = interior_cell(g, t, i, j, k);
     if (active) {
         const long long c = gidx(g, i, j, k);
-        chem_step_cell<NS, false>(g, mech_ref<MT>(mp), c, dtm, 1, CS, prim, T, E, C, CMS, keep_cms);
+        chem_step_cell<NS, false>(g, mech_view<MT>(mp, eu3d_th), c, dtm, 1, CS, prim, T, E, C, CMS, keep_cms);
         nchem[c] = 1;
     }
     const unsigned long long m = __ballot(active);
@@ -1327,6 +1360,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(EU3
                double *__restrict__ prim, double *__restrict__ T, double *__restrict__ E, double *__restrict__ C,
                double *__restrict__ CMS, const int *__restrict__ nchem, int keep_cms)
 {
+    EU3D_THERMO_LDS_STAGE(NS, MT, mp)
     long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
     int i, j, k;
     if (!interior_cell(g, t, i, j, k))
@@ -1334,7 +1368,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(EU3
     long long c = gidx(g, i, j, k);
     int n = nchem[c];
     if (n > 1)
-        chem_rest<NS>(g, mech_ref<MT>(mp), c, n, dtm, CS, prim, T, E, C, CMS, keep_cms);
+        chem_rest<NS>(g, mech_view<MT>(mp, eu3d_th), c, n, dtm, CS, prim, T, E, C, CMS, keep_cms);
 }
 
 // ---------------------------------------------------------------------------
