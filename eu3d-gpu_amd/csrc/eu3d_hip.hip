@@ -250,15 +250,26 @@ __device__ __forceinline__ decltype(auto) mech_view(const DevMech *p, const doub
 // ---------------------------------------------------------------------------
 // ghost fill (src/Flowfield.cpp:593-807)
 // ---------------------------------------------------------------------------
+// All fields are loaded before any is stored (the same values: each field's
+// load still precedes its own store).  In order, the compiler cannot know
+// that the load of field f+1 does not alias the store of field f, and waits
+// for each load in turn.
 __device__ __forceinline__ void copy_point(const Grid &g, int nprim, double *prim, double *T, double *E,
                                            long long d, long long s, int flip_field)
 {
-    for (int f = 0; f < nprim; f++) {
-        double v = prim[f * g.N + s];
-        prim[f * g.N + d] = (f == flip_field) ? -v : v;
-    }
-    T[d] = T[s];
-    E[d] = E[s];
+    constexpr int kMaxNP = 7 + kMaxNS;
+    double v[kMaxNP];
+#pragma unroll
+    for (int f = 0; f < kMaxNP; f++)
+        if (f < nprim)
+            v[f] = prim[f * g.N + s];
+    const double tv = T[s], ev = E[s];
+#pragma unroll
+    for (int f = 0; f < kMaxNP; f++)
+        if (f < nprim)
+            prim[f * g.N + d] = (f == flip_field) ? -v[f] : v[f];
+    T[d] = tv;
+    E[d] = ev;
 }
 
 // A ghost point set to the inflow state (nprim primitives, T, E).
