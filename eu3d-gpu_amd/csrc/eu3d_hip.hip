@@ -420,6 +420,40 @@ __global__ void __launch_bounds__(kBlock) k_ghost_z(Grid g, int nprim, double *p
     copy_point(g, nprim, prim, T, E, gidx(g, i, j, k), gidx(g, i, j, ks), 4);
 }
 
+// z faces for bc = 2 and an even NK (every field's rows then start on 16-byte
+// boundaries): two lanes per k-row, the front (k = 0, 1 from 3, 2) and the
+// back (k = NK-2, NK-1 from NK-3, NK-4), each moving its two values as one
+// 16-byte load and one 16-byte store per field; the back of row r and the
+// front of row r+1 are neighbouring lanes and contiguous in memory.  Same
+// maps and values as k_ghost_z.
+__global__ void __launch_bounds__(kBlock) k_ghost_z2(Grid g, int nprim, double *prim, double *T, double *E)
+{
+    long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (t >= 2LL * g.NI * g.NJ)
+        return;
+    const bool back = t & 1;
+    const long long row = (t >> 1) * g.NK;
+    const long long src = row + (back ? g.NK - 4 : 2), dst = row + (back ? g.NK - 2 : 0);
+    constexpr int kMaxNP = 7 + kMaxNS;
+    double2 v[kMaxNP];
+#pragma unroll
+    for (int f = 0; f < kMaxNP; f++)
+        if (f < nprim)
+            v[f] = *reinterpret_cast<const double2 *>(prim + f * g.N + src);
+    const double2 tv = *reinterpret_cast<const double2 *>(T + src);
+    const double2 ev = *reinterpret_cast<const double2 *>(E + src);
+#pragma unroll
+    for (int f = 0; f < kMaxNP; f++)
+        if (f < nprim) {
+            double2 o; // reversed order; w (field 4) flips sign
+            o.x = (f == 4) ? -v[f].y : v[f].y;
+            o.y = (f == 4) ? -v[f].x : v[f].x;
+            *reinterpret_cast<double2 *>(prim + f * g.N + dst) = o;
+        }
+    *reinterpret_cast<double2 *>(T + dst) = make_double2(tv.y, tv.x);
+    *reinterpret_cast<double2 *>(E + dst) = make_double2(ev.y, ev.x);
+}
+
 // ---------------------------------------------------------------------------
 // Fused flux + conservative update.  MODE 0: EE (CS += RHS); 1..3: SSP-RK3
 // stages (1 also saves CS into CS0); 4: the EE right-hand side into CS0 (IMEX).
@@ -2340,8 +2374,13 @@ int eu3d_fill_ghosts(eu3d_ctx *ctx)
     } else {
         EU3D_DISPATCH_NS(ctx->ns, launch_ghost_y, ctx);
     }
-    long long nz = 2LL * g.NI * g.NJ * g.bc;
-    k_ghost_z<<<blocks_for(nz), kBlock, 0, ctx->stream>>>(g, ctx->nprim, ctx->prim, ctx->T, ctx->E);
+    if (g.bc == 2 && g.NK % 2 == 0) {
+        k_ghost_z2<<<blocks_for(2LL * g.NI * g.NJ), kBlock, 0, ctx->stream>>>(g, ctx->nprim, ctx->prim, ctx->T,
+                                                                            ctx->E);
+    } else {
+        long long nz = 2LL * g.NI * g.NJ * g.bc;
+        k_ghost_z<<<blocks_for(nz), kBlock, 0, ctx->stream>>>(g, ctx->nprim, ctx->prim, ctx->T, ctx->E);
+    }
     CK(hipGetLastError());
     return EU3D_OK;
 }

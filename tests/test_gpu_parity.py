@@ -79,6 +79,26 @@ def test_ghost_fill_index_maps_bitexact(tmp_path):
         assert rel_linf(s.field(n), o.field(n)) <= 1e-14, n
 
 
+@pytest.mark.parametrize("nk", [16, 15])
+def test_ghost_fill_z_maps_bitexact(tmp_path, nk):
+    """z mirrors on a state that varies in z (the shock-bubble's initial
+    state), for an even NK (two lanes per k-row, 16-byte moves) and an odd
+    one (one thread per ghost point): bit-exact, w's sign flip included."""
+    note(f"ghost fill: sbi 16x12x{nk}")
+    o, s = pair(cases.sbi(str(tmp_path), ni=16, nj=12, nk=nk))
+    L = eu3d.load()
+    eu3d.check(L.eu3d_fill_ghosts(s.ctx), s.ctx, "fill_ghosts")
+    s.sync()
+    o.field_boundary()
+    for n in ["P", "U", "V", "W", "T", "Yi", "D"]:
+        a, b = s.field(n), o.field(n)
+        assert np.array_equal(a.view(np.int64), b.view(np.int64)), n
+    for n in ["H", "E", "Gamma"]:
+        assert rel_linf(s.field(n), o.field(n)) <= 1e-14, n
+    Y = s.field("Yi")
+    assert np.any(Y[:, :, 0] != Y[:, :, 1]) and np.any(Y[:, :, -1] != Y[:, :, -2])  # the mirrors see structure
+
+
 def test_ghost_fill_odw_bitexact(tmp_path):
     """The oblique-detonation boundary set (inflow x- / y+, slip wall y-,
     mirror x+ and z) on a state with every field varying: bit-exact."""
