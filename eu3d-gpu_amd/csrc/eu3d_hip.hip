@@ -467,8 +467,9 @@ __global__ void __launch_bounds__(kBlock) k_ghost_z2(Grid g, int nprim, double *
 // tile of every primitive field -- rows j0-2 .. j0+rows+1, columns
 // k0-2 .. k0+KW+1, ghost layers included -- in LDS once, and the y and z
 // stencils read from there.  Every face flux is computed once:
-//   x faces: by the owning thread from HBM/L2; the +i face of plane i is
-//            kept in registers as the -i face of plane i+1;
+//   x faces: by the owning thread after the tile barrier, the point of
+//            plane i from the tile and the other three from HBM/L2; the +i
+//            face of plane i is kept in registers as the -i face of plane i+1;
 //   y faces: each owning thread computes the face above its row; the extra
 //            wave computes the faces below the tile; exchanged through LDS;
 //   z faces: each owning thread computes the face at its +k side; the face
@@ -540,6 +541,20 @@ struct TileLd {
     const double *tile;
     int base, st;
     __device__ __forceinline__ double operator()(int f, int o) const { return tile[f * FS + base + o * st]; }
+};
+
+// x stencil of the face between planes i and i+1: the point of plane i from
+// the LDS tile (the thread's own cell), the others from HBM/L2.
+template <int FS>
+struct XLd {
+    const double *__restrict__ prim;
+    long long N, a, st;
+    const double *tile;
+    int tb;
+    __device__ __forceinline__ double operator()(int f, int o) const
+    {
+        return o == 0 ? tile[f * FS + tb] : prim[(long long)f * N + a + o * st];
+    }
 };
 
 // Tile loads of one thread: the KW interior columns (2 .. KW+1) of every
@@ -657,18 +672,6 @@ __global__ void __launch_bounds__(kFluxThreads) __attribute__((amdgpu_waves_per_
         face_flux<NS, ORDER>(0, prim, N, gidx(g, i0 - 1, jc, kc), g.sI, Fxm);
     for (int i = i0; i < i1; i++) {
         const long long c = gidx(g, i, jc, kc);
-        if (own) {
-            double Fxp[NV];
-            long long ax = c;
-            asm volatile("" : "+v"(ax)); // addressed per plane, not strength-reduced
-            face_flux<NS, ORDER>(0, prim, N, ax, g.sI, Fxp);
-            double cx = -dt / (xn[i] - xn[i - 1]);
-#pragma unroll
-            for (int q = 0; q < NV; q++) {
-                rhs[q] = cx * (Fxp[q] - Fxm[q]);
-                Fxm[q] = Fxp[q];
-            }
-        }
         double u0[NV];
         if (own && MODE != 4) {
 #pragma unroll
@@ -679,6 +682,18 @@ __global__ void __launch_bounds__(kFluxThreads) __attribute__((amdgpu_waves_per_
         // LDS addresses are loop-invariant; formed here, not held across the loop
         int ybase = (ys + 1) * TC + ycol + 2, zbase = (r + 2) * TC + kk + 2, hbase = (l + 2) * TC + 1;
         asm volatile("" : "+v"(ybase), "+v"(zbase), "+v"(hbase));
+        if (own) {
+            double Fxp[NV];
+            long long ax = c;
+            asm volatile("" : "+v"(ax)); // addressed per plane, not strength-reduced
+            face_flux_ld<NS, ORDER>(0, XLd<FS>{prim, N, ax, g.sI, tile, zbase}, Fxp);
+            double cx = -dt / (xn[i] - xn[i - 1]);
+#pragma unroll
+            for (int q = 0; q < NV; q++) {
+                rhs[q] = cx * (Fxp[q] - Fxm[q]);
+                Fxm[q] = Fxp[q];
+            }
+        }
         if (yact) {
             double F[NV];
             face_flux_ld<NS, ORDER>(1, TileLd<FS>{tile, ybase, TC}, F);
