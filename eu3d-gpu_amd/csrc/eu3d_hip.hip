@@ -1017,7 +1017,11 @@ __device__ __forceinline__ int chem_step_cell(const Grid &g, const MT &m, long l
 // reference evaluates them twice on identical inputs).  Cells with Nchem > 1
 // are appended to the stiff queue.  SUR: cells with Nchem >= sur_min are not
 // advanced; they are appended to sur_list for the network.
-template <int NS, class MT, bool SUR>
+// REC: the primitives are those before the advection; each cell first
+// recovers its state from CS (Update_after_CS, as k_recover) and stores it,
+// then runs the chemistry on it (read back from L2): one launch and one CS
+// read fewer than k_recover followed by this kernel, the same arithmetic.
+template <int NS, class MT, bool SUR, bool REC = false>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(EU3D_CHEM_WAVES_PER_EU)))
     k_chem_first(Grid g, const DevMech *__restrict__ mp, double dtm, double *__restrict__ CS,
                  double *__restrict__ prim, double *__restrict__ T, double *__restrict__ E, double *__restrict__ C,
@@ -1032,6 +1036,18 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(EU3
     long long c = active ? gidx(g, i, j, k) : 0;
     int n = 0;
     bool stiff = false, sur = false;
+    if (REC && active) {
+        const long long N = g.N;
+        double cs[NS + 4];
+#pragma unroll
+        for (int q = 0; q < NS + 4; q++)
+            cs[q] = CS[q * N + c];
+        CellState<NS> st;
+        recover<NS>(m, cs, T[c], E[c], st);
+        long long cw = c; // store addresses formed here, not held through the Newton
+        asm volatile("" : "+v"(cw) : "v"(st.T));
+        store_state<NS>(N, cw, st, prim, T, E, C);
+    }
     if (active) {
         n = chem_step_cell<NS, true>(g, m, c, dtm, 0, CS, prim, T, E, C, CMS, keep_cms, SUR ? sur_min : 0,
                                      sur_tmin);
@@ -1907,21 +1923,26 @@ static void sort_queue(eu3d_ctx *ctx)
 }
 
 template <int NS, class MT>
-static void launch_chem(eu3d_ctx *ctx, double dt, int flags)
+static void launch_chem(eu3d_ctx *ctx, double dt, int flags, bool rec)
 {
     const Grid &g = ctx->g;
     int compact = (flags & EU3D_CHEM_COMPACT) ? 1 : 0;
     int keep = (flags & EU3D_CHEM_KEEP_CMS) ? 1 : 0;
     k_chem_reset<<<1, 1, 0, ctx->stream>>>(ctx->cnt);
-    if (flags & EU3D_CHEM_FIXED) {
+    if (rec) {
+        k_chem_first<NS, MT, false, true><<<blocks_for(ctx->n_int), kBlock, 0, ctx->stream>>>(
+            g, ctx->mech, dt, ctx->CS, ctx->prim, ctx->T, ctx->E, ctx->C, ctx->CMS, ctx->nchem, ctx->queue,
+            ctx->cnt, compact, keep, 0, nullptr);
+    } else if (flags & EU3D_CHEM_FIXED) {
         k_chem_fixed<NS, MT><<<blocks_for(ctx->n_int), kBlock, 0, ctx->stream>>>(
             g, ctx->mech, dt, ctx->CS, ctx->prim, ctx->T, ctx->E, ctx->C, ctx->CMS, ctx->nchem, ctx->cnt, keep);
         k_chem_health<<<blocks_for(ctx->n_int), kBlock, 0, ctx->stream>>>(g, ctx->prim, ctx->T, ctx->cnt);
         return;
+    } else {
+        k_chem_first<NS, MT, false><<<blocks_for(ctx->n_int), kBlock, 0, ctx->stream>>>(
+            g, ctx->mech, dt, ctx->CS, ctx->prim, ctx->T, ctx->E, ctx->C, ctx->CMS, ctx->nchem, ctx->queue,
+            ctx->cnt, compact, keep, 0, nullptr);
     }
-    k_chem_first<NS, MT, false><<<blocks_for(ctx->n_int), kBlock, 0, ctx->stream>>>(
-        g, ctx->mech, dt, ctx->CS, ctx->prim, ctx->T, ctx->E, ctx->C, ctx->CMS, ctx->nchem, ctx->queue, ctx->cnt,
-        compact, keep, 0, nullptr);
     if (compact) {
         sort_queue(ctx);
         k_chem_queue<NS, MT><<<ctx->ncu * 4, kBlock, 0, ctx->stream>>>(g, ctx->mech, dt, ctx->CS, ctx->prim, ctx->T,
@@ -2500,7 +2521,8 @@ int eu3d_recover_state(eu3d_ctx *ctx)
     return EU3D_OK;
 }
 
-int eu3d_advance_rk3(eu3d_ctx *ctx, double dt)
+// last_recover false: the recovery after stage 3 is left to the chemistry.
+static int advance_rk3(eu3d_ctx *ctx, double dt, bool last_recover)
 {
     if (!ctx)
         return EU3D_ERR_ARG;
@@ -2516,17 +2538,23 @@ int eu3d_advance_rk3(eu3d_ctx *ctx, double dt)
             Timer tm(ctx, K_FLUX);
             EU3D_DISPATCH_NS(ctx->ns, launch_flux, ctx, stage, dt);
         }
-        rc = eu3d_recover_state(ctx);
-        if (rc)
-            return rc;
+        if (stage < 3 || last_recover) {
+            rc = eu3d_recover_state(ctx);
+            if (rc)
+                return rc;
+        }
     }
     CK(hipGetLastError());
     return EU3D_OK;
 }
 
-static int chemistry_dispatch(eu3d_ctx *ctx, double dt, int flags);
+int eu3d_advance_rk3(eu3d_ctx *ctx, double dt) { return advance_rk3(ctx, dt, true); }
 
-int eu3d_chemistry(eu3d_ctx *ctx, double dt, int flags)
+static int chemistry_dispatch(eu3d_ctx *ctx, double dt, int flags, bool rec);
+
+// rec: the state recovery after the advection is still to do (eu3d_step
+// folds it into the first chemistry pass of the adaptive FP64 path).
+static int chemistry_impl(eu3d_ctx *ctx, double dt, int flags, bool rec)
 {
     if (!ctx)
         return EU3D_ERR_ARG;
@@ -2536,7 +2564,7 @@ int eu3d_chemistry(eu3d_ctx *ctx, double dt, int flags)
         FAIL(EU3D_ERR_ARG, "dt must be > 0");
     if (flags & EU3D_CHEM_SKIP)
         return EU3D_OK;
-    int rc = chemistry_dispatch(ctx, dt, flags);
+    int rc = chemistry_dispatch(ctx, dt, flags, rec);
     if (rc)
         return rc;
     k_log_max_nchem<<<1, 1, 0, ctx->stream>>>(ctx->cnt, ctx->nchem_log + ctx->log_written % kNchemLog);
@@ -2544,6 +2572,8 @@ int eu3d_chemistry(eu3d_ctx *ctx, double dt, int flags)
     CK(hipGetLastError());
     return EU3D_OK;
 }
+
+int eu3d_chemistry(eu3d_ctx *ctx, double dt, int flags) { return chemistry_impl(ctx, dt, flags, false); }
 
 int eu3d_max_nchem_log(eu3d_ctx *ctx, int *out, int capacity, int *count)
 {
@@ -2563,8 +2593,10 @@ int eu3d_max_nchem_log(eu3d_ctx *ctx, int *out, int capacity, int *count)
     return EU3D_OK;
 }
 
-static int chemistry_dispatch(eu3d_ctx *ctx, double dt, int flags)
+static int chemistry_dispatch(eu3d_ctx *ctx, double dt, int flags, bool rec)
 {
+    if (rec && (flags & (EU3D_CHEM_SURROGATE | EU3D_CHEM_FIXED)))
+        FAIL(EU3D_ERR_ARG, "internal: the folded recovery serves the adaptive FP64 path only");
     if ((flags & EU3D_CHEM_SURROGATE) && (flags & EU3D_CHEM_FIXED))
         FAIL(EU3D_ERR_ARG, "EU3D_CHEM_SURROGATE applies to the adaptive trapezoid only");
     if (flags & EU3D_CHEM_SURROGATE) {
@@ -2579,7 +2611,7 @@ static int chemistry_dispatch(eu3d_ctx *ctx, double dt, int flags)
             return rc;
     } else {
         Timer tm(ctx, K_CHEM);
-        EU3D_DISPATCH_NS(ctx->ns, launch_chem, ctx, dt, flags);
+        EU3D_DISPATCH_NS(ctx->ns, launch_chem, ctx, dt, flags, rec);
     }
     CK(hipGetLastError());
     return EU3D_OK;
@@ -2766,22 +2798,24 @@ int eu3d_step(eu3d_ctx *ctx, int scheme, double dt, int chem_flags)
     if (!ctx)
         return EU3D_ERR_ARG;
     int rc;
+    // the adaptive FP64 chemistry recovers the post-advection state itself
+    const bool fold = !(chem_flags & (EU3D_CHEM_SKIP | EU3D_CHEM_SURROGATE | EU3D_CHEM_FIXED));
     if (scheme == EU3D_SCHEME_EE) {
         if ((rc = eu3d_fill_ghosts(ctx)))
             return rc;
         if ((rc = eu3d_flux_update(ctx, dt)))
             return rc;
-        if ((rc = eu3d_recover_state(ctx)))
+        if (!fold && (rc = eu3d_recover_state(ctx)))
             return rc;
     } else if (scheme == EU3D_SCHEME_TVD_RK3) {
-        if ((rc = eu3d_advance_rk3(ctx, dt)))
+        if ((rc = advance_rk3(ctx, dt, !fold)))
             return rc;
     } else if (scheme == EU3D_SCHEME_IMEX) {
         return eu3d_imex_step(ctx, dt, chem_flags);
     } else {
         FAIL(EU3D_ERR_ARG, "unknown time-advance scheme");
     }
-    return eu3d_chemistry(ctx, dt, chem_flags);
+    return chemistry_impl(ctx, dt, chem_flags, fold);
 }
 
 int eu3d_imex_step(eu3d_ctx *ctx, double dt, int chem_flags)

@@ -297,6 +297,38 @@ def test_compacted_equals_uncompacted(tmp_path):
     assert np.array_equal(a.nchem(), b.nchem())
 
 
+@pytest.mark.parametrize("scheme", ["EE", "TVD_RK3"])
+def test_step_folded_recovery_equals_separate_calls(tmp_path, scheme):
+    """eu3d_step folds the recovery after the advection into the first
+    chemistry pass; the same steps through the separate entry points (EE:
+    flux, eu3d_recover_state; RK3: eu3d_advance_rk3, whose stages recover;
+    then eu3d_chemistry) give the same bits."""
+    if scheme == "EE":
+        ctrl, dt, sch = cases.rsbi(str(tmp_path), steps=6, T_pre=1200.0, dt=3e-7), 3e-7, eu3d.SCHEME_EE
+    else:
+        ctrl, dt, sch = cases.sbi(str(tmp_path), ni=24, nj=12, nk=16), 2e-8, eu3d.SCHEME_TVD_RK3
+    L = eu3d.load()
+    a = eu3d.Solver(ctrl)
+    b = eu3d.Solver(ctrl)
+    flags = eu3d.CHEM_COMPACT | eu3d.CHEM_KEEP_CMS
+    for _ in range(6):
+        eu3d.check(L.eu3d_step(a.ctx, sch, dt, flags), a.ctx, "step")
+        if sch == eu3d.SCHEME_EE:
+            eu3d.check(L.eu3d_fill_ghosts(b.ctx), b.ctx, "ghosts")
+            eu3d.check(L.eu3d_flux_update(b.ctx, dt), b.ctx, "flux")
+            eu3d.check(L.eu3d_recover_state(b.ctx), b.ctx, "recover")
+        else:  # the three stages, each followed by its recovery
+            eu3d.check(L.eu3d_advance_rk3(b.ctx, dt), b.ctx, "rk3")
+        eu3d.check(L.eu3d_chemistry(b.ctx, dt, flags), b.ctx, "chemistry")
+    a.sync()
+    b.sync()
+    if sch == eu3d.SCHEME_EE:
+        assert a.interior(a.nchem()).max() > 1
+    for n in ["CS", "T", "E", "P", "D", "U", "V", "W", "Yi", "CMS", "H", "Gamma"]:
+        assert np.array_equal(a.field(n).view(np.int64), b.field(n).view(np.int64)), n
+    assert np.array_equal(a.nchem(), b.nchem())
+
+
 def test_sbi_rk3_cfl(tmp_path):
     """Spherical shock-bubble, SSP-RK3, CFL 0.5, 100 steps: dt sequence and
     state (the 100-step bar, 1e-8)."""
