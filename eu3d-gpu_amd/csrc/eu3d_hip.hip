@@ -92,6 +92,7 @@ struct eu3d_ctx {
     // kNchemLog entries read by eu3d_max_nchem_log
     int *nchem_log;
     long long log_written, log_read;
+    int *log_slot = nullptr; // during a chemistry call: its entry of nchem_log
     // ghost-fill set (EU3D_BOUNDARY_*) and the inflow state of the ODW set
     int boundary;
     double *inflow; // nprim + 2: P D U V W H Gamma Y.. T E
@@ -1103,9 +1104,14 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(EU3
 // resident_health_kernel (src/ReactionCuda.cu:2265-2273): cells whose density
 // or temperature is not finite after the chemistry.
 __global__ void __launch_bounds__(kBlock)
-    k_chem_health(Grid g, const double *__restrict__ prim, const double *__restrict__ T, ChemCounters *cnt)
+    k_chem_health(Grid g, const double *__restrict__ prim, const double *__restrict__ T, ChemCounters *cnt,
+                  int *nchem_slot)
 {
     long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    // Reaction::PushNchemMax (src/Reaction.cpp:953-957): the call's max Nchem,
+    // final here (every chemistry kernel of the call ran before this one)
+    if (nchem_slot && t == 0)
+        *nchem_slot = cnt->max_nchem;
     int i, j, k;
     bool bad = false;
     if (interior_cell(g, t, i, j, k)) {
@@ -1118,9 +1124,6 @@ __global__ void __launch_bounds__(kBlock)
         atomicAdd(&cnt->sticky_bad, (unsigned long long)__popcll(m));
     }
 }
-
-// Reaction::PushNchemMax (src/Reaction.cpp:953-957): the call's max Nchem
-__global__ void k_log_max_nchem(const ChemCounters *cnt, int *slot) { *slot = cnt->max_nchem; }
 
 // Remaining sub-steps 1..Nchem-1 of one cell.
 template <int NS, class MT>
@@ -1936,7 +1939,8 @@ static void launch_chem(eu3d_ctx *ctx, double dt, int flags, bool rec)
     } else if (flags & EU3D_CHEM_FIXED) {
         k_chem_fixed<NS, MT><<<blocks_for(ctx->n_int), kBlock, 0, ctx->stream>>>(
             g, ctx->mech, dt, ctx->CS, ctx->prim, ctx->T, ctx->E, ctx->C, ctx->CMS, ctx->nchem, ctx->cnt, keep);
-        k_chem_health<<<blocks_for(ctx->n_int), kBlock, 0, ctx->stream>>>(g, ctx->prim, ctx->T, ctx->cnt);
+        k_chem_health<<<blocks_for(ctx->n_int), kBlock, 0, ctx->stream>>>(g, ctx->prim, ctx->T, ctx->cnt,
+                                                                          ctx->log_slot);
         return;
     } else {
         k_chem_first<NS, MT, false><<<blocks_for(ctx->n_int), kBlock, 0, ctx->stream>>>(
@@ -1953,7 +1957,8 @@ static void launch_chem(eu3d_ctx *ctx, double dt, int flags, bool rec)
                                                                            ctx->T, ctx->E, ctx->C, ctx->CMS,
                                                                            ctx->nchem, keep);
     }
-    k_chem_health<<<blocks_for(ctx->n_int), kBlock, 0, ctx->stream>>>(g, ctx->prim, ctx->T, ctx->cnt);
+    k_chem_health<<<blocks_for(ctx->n_int), kBlock, 0, ctx->stream>>>(g, ctx->prim, ctx->T, ctx->cnt,
+                                                                          ctx->log_slot);
 }
 
 // Chemistry with the surrogate: cells with Nchem >= sur_min go through the
@@ -1997,7 +2002,8 @@ static void launch_chem_sur(eu3d_ctx *ctx, double dt, int flags, int *rc)
     sort_queue(ctx);
     k_chem_queue<NS, MT, true><<<ctx->ncu * 4, kBlock, 0, ctx->stream>>>(
         g, ctx->mech, dt, ctx->CS, ctx->prim, ctx->T, ctx->E, ctx->C, ctx->CMS, ctx->nchem, ctx->queue2, ctx->cnt, keep);
-    k_chem_health<<<blocks_for(ctx->n_int), kBlock, 0, ctx->stream>>>(g, ctx->prim, ctx->T, ctx->cnt);
+    k_chem_health<<<blocks_for(ctx->n_int), kBlock, 0, ctx->stream>>>(g, ctx->prim, ctx->T, ctx->cnt,
+                                                                          ctx->log_slot);
 }
 
 template <int NS, class MT>
@@ -2564,10 +2570,11 @@ static int chemistry_impl(eu3d_ctx *ctx, double dt, int flags, bool rec)
         FAIL(EU3D_ERR_ARG, "dt must be > 0");
     if (flags & EU3D_CHEM_SKIP)
         return EU3D_OK;
+    ctx->log_slot = ctx->nchem_log + ctx->log_written % kNchemLog; // written by the call's k_chem_health
     int rc = chemistry_dispatch(ctx, dt, flags, rec);
+    ctx->log_slot = nullptr;
     if (rc)
         return rc;
-    k_log_max_nchem<<<1, 1, 0, ctx->stream>>>(ctx->cnt, ctx->nchem_log + ctx->log_written % kNchemLog);
     ctx->log_written++;
     CK(hipGetLastError());
     return EU3D_OK;
@@ -2837,7 +2844,8 @@ int eu3d_imex_step(eu3d_ctx *ctx, double dt, int chem_flags)
         Timer tm(ctx, K_CHEM);
         k_chem_reset<<<1, 1, 0, ctx->stream>>>(ctx->cnt);
         EU3D_DISPATCH_NS(ctx->ns, launch_imex, ctx, dt, (chem_flags & EU3D_CHEM_KEEP_CMS) ? 1 : 0);
-        k_chem_health<<<blocks_for(ctx->n_int), kBlock, 0, ctx->stream>>>(ctx->g, ctx->prim, ctx->T, ctx->cnt);
+        k_chem_health<<<blocks_for(ctx->n_int), kBlock, 0, ctx->stream>>>(ctx->g, ctx->prim, ctx->T, ctx->cnt,
+                                                                          nullptr);
     }
     CK(hipGetLastError());
     return EU3D_OK;
