@@ -205,3 +205,28 @@ def test_python_constants_match_the_headers():
             assert py == int(val, 0), (h, name, py, val)
             seen += 1
     assert seen >= 15
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py --gpus N with N > 1 exits with a message instead of running."""
+    import subprocess
+    import sys
+
+    r = subprocess.run([sys.executable, os.path.join(REPO, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode != 0
+    assert "one GPU" in (r.stdout + r.stderr)
+
+
+def test_missing_library_fails_loudly(tmp_path):
+    """Without libeu3d.so the binding raises; nothing falls back to the oracle."""
+    import subprocess
+    import sys
+
+    code = ("import importlib, sys; sys.path.insert(0, %r); m = importlib.import_module('eu3d-gpu_amd'); "
+            "m.LIB_PATH = %r\n"
+            "try:\n    m.load()\nexcept RuntimeError as e:\n    print('raised', e)\n    sys.exit(3)\n"
+            "sys.exit(0)") % (REPO, str(tmp_path / "libeu3d.so"))
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 3, r.stdout + r.stderr
+    assert "missing" in r.stdout
