@@ -75,6 +75,8 @@ struct eu3d_ctx {
     long long n_int;
     hipStream_t stream;
     DevMech hmech;
+    double cfl_hx = 0, cfl_hy = 0;      // x[1]-x[0], y[1]-y[0] of the node coordinates (fixed per context)
+    unsigned long long *cfl_host = nullptr; // pinned host word for the CFL minimum
     DevMech *mech;
     double *x, *y, *z;
     double *prim, *T, *E, *C, *CS, *CS0, *CMS, *scratch;
@@ -2181,6 +2183,8 @@ int eu3d_destroy(eu3d_ctx *ctx)
     for (void *p : ptrs)
         if (p)
             (void)hipFree(p);
+    if (ctx->cfl_host)
+        (void)hipHostFree(ctx->cfl_host);
     if (ctx->stream)
         (void)hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -2268,6 +2272,9 @@ int eu3d_create(const eu3d_mesh *mesh, const eu3d_mechanism *mc, eu3d_ctx **out)
     CK(hipMalloc(&ctx->z, sizeof(double) * g.NK));
     CK(hipMemcpy(ctx->x, mesh->xnode, sizeof(double) * g.NI, hipMemcpyHostToDevice));
     CK(hipMemcpy(ctx->y, mesh->ynode, sizeof(double) * g.NJ, hipMemcpyHostToDevice));
+    ctx->cfl_hx = mesh->xnode[1] - mesh->xnode[0];
+    ctx->cfl_hy = mesh->ynode[1] - mesh->ynode[0];
+    CK(hipHostMalloc((void **)&ctx->cfl_host, sizeof(unsigned long long), hipHostMallocDefault));
     CK(hipMemcpy(ctx->z, mesh->znode, sizeof(double) * g.NK, hipMemcpyHostToDevice));
     size_t N = (size_t)g.N;
     CK(hipMalloc(&ctx->prim, sizeof(double) * N * ctx->nprim));
@@ -2465,19 +2472,9 @@ int eu3d_cfl_dt(eu3d_ctx *ctx, double cfl, double time, double final_time, doubl
         return EU3D_ERR_ARG;
     if (!ctx->has_state)
         FAIL(EU3D_ERR_STATE, "no state uploaded");
-    double hx, hy;
-    CK(hipMemcpy(&hx, ctx->x, sizeof(double), hipMemcpyDeviceToHost));
-    {
-        double x1;
-        CK(hipMemcpy(&x1, ctx->x + 1, sizeof(double), hipMemcpyDeviceToHost));
-        hx = x1 - hx;
-    }
-    CK(hipMemcpy(&hy, ctx->y, sizeof(double), hipMemcpyDeviceToHost));
-    {
-        double y1;
-        CK(hipMemcpy(&y1, ctx->y + 1, sizeof(double), hipMemcpyDeviceToHost));
-        hy = y1 - hy;
-    }
+    // the spacings were formed at eu3d_create from the same coordinates
+    // (reading them here would block on the device four times per step)
+    const double hx = ctx->cfl_hx, hy = ctx->cfl_hy;
     unsigned long long init;
     double big = 1e10;
     std::memcpy(&init, &big, sizeof init);
@@ -2489,9 +2486,9 @@ int eu3d_cfl_dt(eu3d_ctx *ctx, double cfl, double time, double final_time, doubl
         k_cfl<<<nb, kBlock, 0, ctx->stream>>>(ctx->g, ctx->prim, ctx->C, hx, hy, bits);
         CK(hipGetLastError());
     }
-    unsigned long long res;
-    CK(hipMemcpyAsync(&res, bits, sizeof res, hipMemcpyDeviceToHost, ctx->stream));
+    CK(hipMemcpyAsync(ctx->cfl_host, bits, sizeof(unsigned long long), hipMemcpyDeviceToHost, ctx->stream));
     CK(hipStreamSynchronize(ctx->stream));
+    const unsigned long long res = *ctx->cfl_host;
     double dt_conv;
     std::memcpy(&dt_conv, &res, sizeof dt_conv);
     double d = cfl * dt_conv;
