@@ -1217,7 +1217,8 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(EU3
     long long c = -1;
     int n = 0, step = 0; // the cell is done when step >= n
     bool fresh = false;  // FB: sub-step 0 (with the Nchem prediction) still to do
-    bool more = true;    // wave-uniform: the queue may still hold cells
+    bool more = total > 0; // wave-uniform: the queue may still hold cells
+    bool first = true;     // wave-uniform: no fetch yet
     for (;;) {
         const bool need = step >= n;
         if (more) {
@@ -1225,8 +1226,16 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(EU3
             if (want) {
                 const int leader = __ffsll((long long)want) - 1;
                 unsigned int base = 0;
-                if ((int)(threadIdx.x & 63) == leader)
-                    base = atomicAdd(&cnt->fetch, (unsigned int)__popcll(want));
+                if ((int)(threadIdx.x & 63) == leader) {
+                    // first fetch: a plain read first, so that once a short
+                    // queue is handed out the remaining waves leave without
+                    // contending for the counter; later fetches go straight
+                    // to the atomic (a long queue keeps its one round trip)
+                    base = first ? __hip_atomic_load(&cnt->fetch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0u;
+                    if (base < total)
+                        base = atomicAdd(&cnt->fetch, (unsigned int)__popcll(want));
+                }
+                first = false;
                 base = __shfl(base, leader, 64);
                 if (base + (unsigned int)__popcll(want) >= total)
                     more = false;
