@@ -219,6 +219,28 @@ def test_rates_probe_dead_reactions(tmp_path, generic):
     assert zeros >= len(temps) * ns  # the inert set alone is all zeros
 
 
+def test_dead_reaction_window_off_for_overflowing_rates(tmp_path):
+    """A mechanism whose forward rate constant overflows inside 200-6000 K
+    (reaction 10 with Ea = -5e5 cal/mol: Ea/(Ru T) = 1258 at 200 K) fails the
+    finiteness check at eu3d_create, so no reaction is skipped there."""
+    src = os.path.join(os.path.dirname(cases.__file__), "..", "eu3d-gpu_amd", "input", "H2_JA919.txt")
+    text = open(src).read()
+    assert text.count("-0.180099690E4") == 1
+    mech = tmp_path / "H2_overflow.txt"
+    mech.write_text(text.replace("-0.180099690E4", "-5.0E5"))
+    ctrl = cases.rsbi(str(tmp_path))
+    c = open(ctrl).read()
+    c = "\n".join(f"reaction-model {mech}" if l.startswith("reaction-model") else l for l in c.splitlines()) + "\n"
+    open(ctrl, "w").write(c)
+    L = eu3d.load()
+    s = eu3d.Solver(ctrl)
+    v = ctypes.c_int(-1)
+    eu3d.check(L.eu3d_get_option(s.ctx, eu3d.OPT_DEAD_REACTION_SKIP, ctypes.byref(v)), s.ctx)
+    assert v.value == 0
+    eu3d.check(L.eu3d_get_option(s.ctx, eu3d.OPT_GENERATED_AVAILABLE, ctypes.byref(v)), s.ctx)
+    assert v.value == 0  # the tables differ from the compiled-in mechanism
+
+
 @pytest.mark.parametrize("direction", [1, 2, 3])
 def test_face_flux_matches_oracle(tmp_path, direction):
     """MUSCL_2 + AUSM face fluxes of one direction from the same ghost-filled state."""
