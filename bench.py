@@ -28,8 +28,9 @@ FP64_PEAK_TFLOPS = 78.6   # MI355X FP64 (vector and matrix)
 
 # Algorithmic HBM bytes of the fused flux/update kernel per interior node and
 # RK stage: read 16 primitive fields + 13 conserved, write 13 conserved
-# (42 doubles).  RK stages 2 and 3 also read the stage-0 copy (13 doubles)
-# and stage 1 writes it: +104 B each.  Ghost/halo re-reads are overhead.
+# (42 doubles).  RK stages 2 and 3 also read u^n (13 doubles): +104 B each;
+# stage 1 writes u1 into the second buffer, so there is no extra copy.
+# Ghost/halo re-reads are overhead.
 FLUX_BYTES_PER_NODE = 336
 RK_EXTRA_BYTES_PER_NODE = 104
 

@@ -5,7 +5,7 @@
 //
 // Device layout (structure of arrays over the padded grid, k fastest):
 //   prim[f*N + idx], f = P D U V W H Gamma Y0..Y(ns-1)
-//   T, E, C [N];  CS[n*N + idx], n < ns+4;  CS0 (RK3 stage 0);  CMS[s*N + idx]
+//   T, E, C [N];  CS[n*N + idx], n < ns+4;  CS0 (RK3 stages 1-2, IMEX RHS);  CMS[s*N + idx]
 //   nchem[N] (int)
 #include <hip/hip_runtime.h>
 
@@ -458,8 +458,13 @@ __global__ void __launch_bounds__(kBlock) k_ghost_z2(Grid g, int nprim, double *
 }
 
 // ---------------------------------------------------------------------------
-// Fused flux + conservative update.  MODE 0: EE (CS += RHS); 1..3: SSP-RK3
-// stages (1 also saves CS into CS0); 4: the EE right-hand side into CS0 (IMEX).
+// Fused flux + conservative update: reads u from Uin, writes Uout.  MODE 0:
+// EE (u + RHS, in place); 1..3: SSP-RK3 stages, u^n stays in CS --
+//   1: u1 = un + RHS(un)                 Uin = CS,  Uout = CS0
+//   2: u2 = 3/4 un + 1/4 (u1 + RHS(u1))  Uin = CS0, Uout = CS0, Aux = CS (un)
+//   3: un+1 = 1/3 un + 2/3 (u2 + RHS)    Uin = CS0, Uout = CS,  Aux = CS (un)
+// (each cell reads its own values before writing them, so the in-place
+// stages are safe); 4: the EE right-hand side alone into Aux (IMEX).
 // RHS = -dt/dx (F+ - F-) - dt/dy (G+ - G-) - dt/dz (Q+ - Q-) in that order
 // (TimeAdv::EE, src/TimeAdv.cpp:51-54).
 //
@@ -637,7 +642,7 @@ __device__ __forceinline__ void tile_store(double *tile, int w, int l, int t,
 
 template <int NS, int MODE, int KW, int ORDER>
 __global__ void __launch_bounds__(kFluxThreads) __attribute__((amdgpu_waves_per_eu(2)))
-    k_flux_update(Grid g, const double *__restrict__ prim, double *__restrict__ CS, double *__restrict__ CS0,
+    k_flux_update(Grid g, const double *__restrict__ prim, const double *Uin, double *Uout, double *Aux,
                   const double *__restrict__ xn, const double *__restrict__ yn, const double *__restrict__ zn,
                   double dt)
 {
@@ -679,7 +684,7 @@ __global__ void __launch_bounds__(kFluxThreads) __attribute__((amdgpu_waves_per_
         if (own && MODE != 4) {
 #pragma unroll
             for (int q = 0; q < NV; q++)
-                u0[q] = CS[q * N + c];
+                u0[q] = Uin[q * N + c];
         }
         __syncthreads(); // tile of plane i complete
         // LDS addresses are loop-invariant; formed here, not held across the loop
@@ -735,23 +740,20 @@ __global__ void __launch_bounds__(kFluxThreads) __attribute__((amdgpu_waves_per_
                 double r1 = rhs[q] - cy * (fy[(q * (ROWS + 1) + r + 1) * KW + kk] - fy[(q * (ROWS + 1) + r) * KW + kk]);
                 double d = (kk == 0) ? zp0[q * ROWS + r] - zb[q * ROWS + r] : dz[q];
                 double rq = r1 - cz * d;
-                if (MODE == 4) { // IMEX: the right-hand side alone, into CS0
-                    CS0[q * N + cw] = rq;
+                if (MODE == 4) { // IMEX: the right-hand side alone
+                    Aux[q * N + cw] = rq;
                     continue;
                 }
                 double u = u0[q];
                 double vv;
-                if (MODE == 0) {
-                    vv = u + rq;
-                } else if (MODE == 1) {
-                    CS0[q * N + cw] = u;
+                if (MODE == 0 || MODE == 1) {
                     vv = u + rq;
                 } else if (MODE == 2) {
-                    vv = 0.75 * CS0[q * N + cw] + 0.25 * (u + rq);
+                    vv = 0.75 * Aux[q * N + cw] + 0.25 * (u + rq);
                 } else {
-                    vv = (1.0 / 3.0) * CS0[q * N + cw] + (2.0 / 3.0) * (u + rq);
+                    vv = (1.0 / 3.0) * Aux[q * N + cw] + (2.0 / 3.0) * (u + rq);
                 }
-                CS[q * N + cw] = vv;
+                Uout[q * N + cw] = vv;
             }
         }
         if (i + 1 < i1) {
@@ -1862,26 +1864,27 @@ static void launch_flux_kw(eu3d_ctx *ctx, int mode, double dt)
 {
     const Grid &g = ctx->g;
     const dim3 grid(flux_blocks(g, KW));
+    double *cs = ctx->CS, *cs1 = ctx->CS0;
     switch (mode) {
     case 0:
-        k_flux_update<NS, 0, KW, ORDER><<<grid, kFluxThreads, 0, ctx->stream>>>(g, ctx->prim, ctx->CS, ctx->CS0, ctx->x,
-                                                                        ctx->y, ctx->z, dt);
+        k_flux_update<NS, 0, KW, ORDER><<<grid, kFluxThreads, 0, ctx->stream>>>(g, ctx->prim, cs, cs, nullptr, ctx->x,
+                                                                                ctx->y, ctx->z, dt);
         break;
     case 1:
-        k_flux_update<NS, 1, KW, ORDER><<<grid, kFluxThreads, 0, ctx->stream>>>(g, ctx->prim, ctx->CS, ctx->CS0, ctx->x,
-                                                                        ctx->y, ctx->z, dt);
+        k_flux_update<NS, 1, KW, ORDER><<<grid, kFluxThreads, 0, ctx->stream>>>(g, ctx->prim, cs, cs1, nullptr, ctx->x,
+                                                                                ctx->y, ctx->z, dt);
         break;
     case 2:
-        k_flux_update<NS, 2, KW, ORDER><<<grid, kFluxThreads, 0, ctx->stream>>>(g, ctx->prim, ctx->CS, ctx->CS0, ctx->x,
-                                                                        ctx->y, ctx->z, dt);
+        k_flux_update<NS, 2, KW, ORDER><<<grid, kFluxThreads, 0, ctx->stream>>>(g, ctx->prim, cs1, cs1, cs, ctx->x,
+                                                                                ctx->y, ctx->z, dt);
         break;
     case 3:
-        k_flux_update<NS, 3, KW, ORDER><<<grid, kFluxThreads, 0, ctx->stream>>>(g, ctx->prim, ctx->CS, ctx->CS0, ctx->x,
-                                                                        ctx->y, ctx->z, dt);
+        k_flux_update<NS, 3, KW, ORDER><<<grid, kFluxThreads, 0, ctx->stream>>>(g, ctx->prim, cs1, cs, cs, ctx->x,
+                                                                                ctx->y, ctx->z, dt);
         break;
     default:
-        k_flux_update<NS, 4, KW, ORDER><<<grid, kFluxThreads, 0, ctx->stream>>>(g, ctx->prim, ctx->CS, ctx->CS0, ctx->x,
-                                                                        ctx->y, ctx->z, dt);
+        k_flux_update<NS, 4, KW, ORDER><<<grid, kFluxThreads, 0, ctx->stream>>>(g, ctx->prim, cs, cs, cs1, ctx->x,
+                                                                                ctx->y, ctx->z, dt);
         break;
     }
 }
@@ -1915,9 +1918,9 @@ static void launch_flux(eu3d_ctx *ctx, int mode, double dt)
 }
 
 template <int NS, class MT>
-static void launch_recover(eu3d_ctx *ctx)
+static void launch_recover(eu3d_ctx *ctx, const double *cs)
 {
-    k_recover<NS, MT><<<blocks_for(ctx->n_int), kBlock, 0, ctx->stream>>>(ctx->g, ctx->mech, ctx->CS, ctx->prim, ctx->T,
+    k_recover<NS, MT><<<blocks_for(ctx->n_int), kBlock, 0, ctx->stream>>>(ctx->g, ctx->mech, cs, ctx->prim, ctx->T,
                                                                        ctx->E, ctx->C);
 }
 
@@ -2521,17 +2524,20 @@ int eu3d_flux_update(eu3d_ctx *ctx, double dt)
     return EU3D_OK;
 }
 
-int eu3d_recover_state(eu3d_ctx *ctx)
+// Update_after_CS from the conserved state in cs (CS, or CS0 inside RK3).
+static int recover_from(eu3d_ctx *ctx, const double *cs)
 {
     if (!ctx)
         return EU3D_ERR_ARG;
     if (!ctx->has_state)
         FAIL(EU3D_ERR_STATE, "no state uploaded");
     Timer tm(ctx, K_RECOVER);
-    EU3D_DISPATCH_NS(ctx->ns, launch_recover, ctx);
+    EU3D_DISPATCH_NS(ctx->ns, launch_recover, ctx, cs);
     CK(hipGetLastError());
     return EU3D_OK;
 }
+
+int eu3d_recover_state(eu3d_ctx *ctx) { return ctx ? recover_from(ctx, ctx->CS) : EU3D_ERR_ARG; }
 
 // last_recover false: the recovery after stage 3 is left to the chemistry.
 static int advance_rk3(eu3d_ctx *ctx, double dt, bool last_recover)
@@ -2551,7 +2557,8 @@ static int advance_rk3(eu3d_ctx *ctx, double dt, bool last_recover)
             EU3D_DISPATCH_NS(ctx->ns, launch_flux, ctx, stage, dt);
         }
         if (stage < 3 || last_recover) {
-            rc = eu3d_recover_state(ctx);
+            // stages 1 and 2 leave their state in CS0, stage 3 in CS
+            rc = recover_from(ctx, stage < 3 ? ctx->CS0 : ctx->CS);
             if (rc)
                 return rc;
         }
