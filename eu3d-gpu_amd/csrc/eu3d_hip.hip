@@ -819,6 +819,7 @@ __global__ void __launch_bounds__(kBlock)
     __shared__ double red[kBlock / 64];
     double best = 1e10;
     long long n_int = (long long)g.ni * g.nj * g.nk;
+#pragma unroll 4
     for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < n_int;
          t += (long long)gridDim.x * blockDim.x) {
         int i, j, k;
