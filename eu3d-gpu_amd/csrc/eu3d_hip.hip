@@ -340,19 +340,22 @@ __global__ void __launch_bounds__(kBlock)
     const auto &m = mech_ref<MT>(mp);
     int j = jj, js = 2 * g.bc - 1 - j;
     long long d = gidx(g, i, j, k), s = gidx(g, i, js, k);
+    // every source value is loaded before any store (see copy_point)
     double Y[NS];
-    prim[2 * N + d] = prim[2 * N + s];
-    prim[3 * N + d] = -prim[3 * N + s];
-    prim[4 * N + d] = prim[4 * N + s];
-    double P = prim[0 * N + s];
+    const double U = prim[2 * N + s], V = prim[3 * N + s], W = prim[4 * N + s];
+    const double P = prim[0 * N + s];
+    const double Tt = T[s];
+#pragma unroll
+    for (int q = 0; q < NS; q++)
+        Y[q] = prim[(7 + q) * N + s];
+    prim[2 * N + d] = U;
+    prim[3 * N + d] = -V;
+    prim[4 * N + d] = W;
     prim[0 * N + d] = P;
-    double Tt = T[s];
     T[d] = Tt;
 #pragma unroll
-    for (int q = 0; q < NS; q++) {
-        Y[q] = prim[(7 + q) * N + s];
+    for (int q = 0; q < NS; q++)
         prim[(7 + q) * N + d] = Y[q];
-    }
     TPow tp = tpow(Tt);
     double sR = 0.0, sCp = 0.0, sH = 0.0, sE = 0.0;
 #pragma unroll
