@@ -291,6 +291,28 @@ def test_rsbi_ee_full_steps(tmp_path, steps, tol):
     assert rc == 0 and h.invalid_nchem == 0 and h.nonfinite == 0
 
 
+def test_rsbi_reference_problem_full_grid(tmp_path):
+    """The reference's problem as shipped (input/ctrl-3dRSBI.txt: 281x141x16,
+    EE, Trapezoid-Adaptive, dt 3e-8 s) at its full size against the oracle
+    (OpenMP build): 1 step within 1e-12, 10 steps within the multi-step bar
+    1e-8 (V, which starts near zero, drifts to ~3e-10 of its maximum), Nchem
+    equal."""
+    base = os.path.join(os.path.dirname(cases.__file__), "..", "eu3d-gpu_amd")
+    ctrl = os.path.join(base, "input", "ctrl-3dRSBI.txt")
+    note("rsbi 281x141x16, 1 + 9 steps")
+    o = oracle.Oracle(ctrl, base_dir=base, omp=True, threads=min(16, os.cpu_count() or 1))
+    s = eu3d.Solver(ctrl, base_dir=base)
+    assert o.run(1) == 1
+    s.run(1)
+    s.sync()
+    compare_state(o, s, 1e-12, "RSBI 281x141x16, 1 step")
+    assert o.run(9) == 9
+    s.run(9)
+    s.sync()
+    compare_state(o, s, 1e-8, "RSBI 281x141x16, 10 steps")
+    assert np.array_equal(s.interior(s.nchem()), o.interior(o.nchem()))
+
+
 def test_rsbi_stiff_chemistry(tmp_path):
     """Hot RSBI with a large step: cells with Nchem > 1 go through the compacted queue."""
     note("rsbi stiff 10 steps")
