@@ -313,6 +313,49 @@ def test_rsbi_reference_problem_full_grid(tmp_path):
     assert np.array_equal(s.interior(s.nchem()), o.interior(o.nchem()))
 
 
+def test_sbi_benchmark_full_grid(tmp_path):
+    """The benchmark's shock-bubble (input/ctrl-3dSBI.txt: 256x128x128,
+    SSP-RK3, CFL 0.5) at its full size against the oracle (OpenMP build):
+    1 step within 1e-12, 3 steps within the multi-step bar 1e-8.  V and W
+    (and the y/z momentum) start at zero and grow from rounding-level pressure
+    imbalances across the pressure-equilibrium bubble, so they are measured
+    against the speed / momentum magnitude; every other field against its own
+    maximum."""
+    base = os.path.join(os.path.dirname(cases.__file__), "..", "eu3d-gpu_amd")
+    ctrl = os.path.join(base, "input", "ctrl-3dSBI.txt")
+    note("sbi 256x128x128, 1 + 2 steps")
+    o = oracle.Oracle(ctrl, base_dir=base, omp=True, threads=min(16, os.cpu_count() or 1))
+    s = eu3d.Solver(ctrl, base_dir=base)
+    transverse = {"V", "W", f"CS{o.ns + 1}", f"CS{o.ns + 2}"}
+
+    def check(tol, what):
+        compare_state(o, s, tol, what, fields=[f for f in SCALARS if f not in transverse], cs=False)
+        a, b = s.interior(s.field("CS")), o.interior(o.field("CS"))
+        for q in range(o.ns + 4):
+            if f"CS{q}" not in transverse:
+                assert rel_linf(a[..., q], b[..., q]) <= tol, (what, q)
+        mom = np.sqrt(sum(b[..., o.ns + d] ** 2 for d in range(3))).max()
+        vel = np.sqrt(sum(o.interior(o.field(n)) ** 2 for n in "UVW")).max()
+        for q in (o.ns + 1, o.ns + 2):
+            err = np.abs(a[..., q] - b[..., q]).max() / mom
+            note(f"{what}: CS{q} error / momentum magnitude = {err:.3e}")
+            assert err <= tol, (what, q)
+        for n in ("V", "W"):
+            err = np.abs(s.interior(s.field(n)) - o.interior(o.field(n))).max() / vel
+            note(f"{what}: {n} error / speed = {err:.3e}")
+            assert err <= tol, (what, n)
+
+    assert o.run(1) == 1
+    s.run(1)
+    s.sync()
+    check(1e-12, "SBI 256x128x128, 1 step")
+    assert o.run(2) == 2
+    s.run(2)
+    s.sync()
+    check(1e-8, "SBI 256x128x128, 3 steps")
+    assert np.array_equal(s.interior(s.nchem()), o.interior(o.nchem()))
+
+
 def test_rsbi_stiff_chemistry(tmp_path):
     """Hot RSBI with a large step: cells with Nchem > 1 go through the compacted queue."""
     note("rsbi stiff 10 steps")
